@@ -1,0 +1,259 @@
+// Complex GEMM on the CDNA matrix cores (MFMA), FP32 and FP64.
+//
+// C = alpha * A op(B) + beta * C with row-major interleaved-complex operands and real
+// alpha, beta. op = 0 (NN): B is K x N. op = 1 (NC): B is N x K and conj(B)^T is
+// used -- the projector contraction <beta|psi> and every subspace overlap is of this
+// form, with the reduction along the contiguous plane-wave axis of both operands.
+//
+// Complex arithmetic runs on the real matrix cores as 2 x 2 real blocks:
+//   C_re += A_re B_re - A_im B_im,  C_im += A_re B_im + A_im B_re
+// i.e. four v_mfma_*_16x16x4 per 16x16x4 complex step (A_im is negated in registers).
+// The FP32 and FP64 instructions share one operand layout, so one template serves
+// both: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and receives
+// D[4 (l / 16) + q][l % 16], q = 0..3.
+//
+// Tiling: a 256-thread workgroup (4 wavefronts) computes a 64 x 64 complex tile; each
+// wavefront a 32 x 32 quarter as 2 x 2 MFMA blocks. K advances 16 at a time through
+// LDS, stored k-major so fragment reads are unit-stride across lanes. Long-K /
+// small-output products (projections onto a few hundred bands) are split along K
+// into enough workgroups to fill 256 CUs, with a deterministic second-pass reduction.
+#include "kernels.hpp"
+
+namespace amt {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename R>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  using acc_t = floatx4;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<double> {
+  using acc_t = doublex4;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int BM = 64, BN = 64, BK = 16;
+
+template <typename R, int OP>
+__global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, int nsplit,
+                                               const typename Cplx<R>::T* __restrict__ A, long long lda,
+                                               long long sA, const typename Cplx<R>::T* __restrict__ B,
+                                               long long ldb, long long sB, typename Cplx<R>::T* __restrict__ Cm,
+                                               long long ldc, long long sC, R alpha, R beta,
+                                               typename Cplx<R>::T* __restrict__ part) {
+  using C = typename Cplx<R>::T;
+  using acc_t = typename Mfma<R>::acc_t;
+  __shared__ C As[BK][BM + 1];
+  __shared__ C Bs[BK][BN + 1];
+  const int bz = blockIdx.z;
+  const int batch = bz / nsplit, split = bz % nsplit;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int kbeg = split * kchunk;
+  const int kend = (kbeg + kchunk < K) ? kbeg + kchunk : K;
+  A += (long long)batch * sA;
+  B += (long long)batch * sB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  acc_t accr[2][2], acci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        accr[a][b][q] = 0;
+        acci[a][b][q] = 0;
+      }
+  const C zero = cmake<C>(0, 0);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    {
+      const int row = tid >> 2, kq = (tid & 3) * 4;
+      const int gm = m0 + row;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int gk = k0 + kq + u;
+        As[kq + u][row] = (gm < M && gk < kend) ? A[(long long)gm * lda + gk] : zero;
+      }
+    }
+    if (OP == 0) {
+      const int kk = tid >> 4, c0 = (tid & 15) * 4;
+      const int gk = k0 + kk;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int gn = n0 + c0 + u;
+        Bs[kk][c0 + u] = (gk < kend && gn < N) ? B[(long long)gk * ldb + gn] : zero;
+      }
+    } else {
+      const int col = tid >> 2, kq = (tid & 3) * 4;
+      const int gn = n0 + col;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int gk = k0 + kq + u;
+        C v = (gn < N && gk < kend) ? B[(long long)gn * ldb + gk] : zero;
+        v.y = -v.y;
+        Bs[kq + u][col] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4);
+      C a[2], b[2];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) a[rb] = As[kr][wr + rb * 16 + (lane & 15)];
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) b[cb] = Bs[kr][wc + cb * 16 + (lane & 15)];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+          accr[rb][cb] = Mfma<R>::mma(a[rb].x, b[cb].x, accr[rb][cb]);
+          accr[rb][cb] = Mfma<R>::mma(-a[rb].y, b[cb].y, accr[rb][cb]);
+          acci[rb][cb] = Mfma<R>::mma(a[rb].x, b[cb].y, acci[rb][cb]);
+          acci[rb][cb] = Mfma<R>::mma(a[rb].y, b[cb].x, acci[rb][cb]);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = m0 + wr + rb * 16 + (lane >> 4) * 4 + q;
+        const int col = n0 + wc + cb * 16 + (lane & 15);
+        if (row < M && col < N) {
+          const C v = cmake<C>(accr[rb][cb][q], acci[rb][cb][q]);
+          if (part) {
+            part[(((long long)batch * nsplit + split) * M + row) * N + col] = v;
+          } else {
+            C* dst = Cm + (long long)batch * sC + (long long)row * ldc + col;
+            C o = cmake<C>(alpha * v.x, alpha * v.y);
+            if (beta != (R)0) {
+              const C old = *dst;
+              o.x += beta * old.x;
+              o.y += beta * old.y;
+            }
+            *dst = o;
+          }
+        }
+      }
+}
+
+template <typename R>
+__global__ void k_splitk_reduce(int M, int N, int nsplit, const typename Cplx<R>::T* __restrict__ part,
+                                typename Cplx<R>::T* __restrict__ Cm, long long ldc, long long sC, R alpha,
+                                R beta) {
+  using C = typename Cplx<R>::T;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int batch = blockIdx.y;
+  if (e >= (long long)M * N) return;
+  const int row = (int)(e / N), col = (int)(e % N);
+  R sr = 0, si = 0;
+  const C* p = part + (long long)batch * nsplit * M * N + e;
+  for (int s = 0; s < nsplit; ++s) {
+    const C v = p[(long long)s * M * N];
+    sr += v.x;
+    si += v.y;
+  }
+  C* dst = Cm + (long long)batch * sC + (long long)row * ldc + col;
+  C o = cmake<C>(alpha * sr, alpha * si);
+  if (beta != (R)0) {
+    const C old = *dst;
+    o.x += beta * old.x;
+    o.y += beta * old.y;
+  }
+  *dst = o;
+}
+
+static int choose_split(int m, int n, int k, int batch) {
+  const long long tiles = (long long)((m + BM - 1) / BM) * ((n + BN - 1) / BN) * batch;
+  if (tiles >= 512) return 1;
+  long long want = (1024 + tiles - 1) / tiles;
+  const long long kmax = (k + 255) / 256;  // keep at least 256 of K per split
+  if (want > kmax) want = kmax;
+  if (want < 1) want = 1;
+  if (want > 256) want = 256;
+  return (int)want;
+}
+
+size_t zgemm_workspace_bytes(int prec, int op, int m, int n, int k, int batch) {
+  (void)op;
+  const int ns = choose_split(m, n, k, batch);
+  if (ns <= 1) return 0;
+  const size_t c = prec == kFP64 ? 16 : 8;
+  return (size_t)batch * ns * m * n * c;
+}
+
+template <typename R>
+static int zgemm_t(int op, int m, int n, int k, double alpha, const void* a, long long lda, long long sa,
+                   const void* b, long long ldb, long long sb, double beta, void* c, long long ldc, long long sc,
+                   int batch, void* ws, size_t ws_bytes, hipStream_t s) {
+  using C = typename Cplx<R>::T;
+  if (m <= 0 || n <= 0 || batch <= 0) return 0;
+  int ns = choose_split(m, n, k, batch);
+  if (k <= 0) ns = 1;
+  const size_t need = ns > 1 ? (size_t)batch * ns * m * n * sizeof(C) : 0;
+  C* part = nullptr;
+  if (ns > 1) {
+    if (ws == nullptr || ws_bytes < need) {
+      set_error("zgemm: split-K workspace too small (" + std::to_string(ws_bytes) + " < " + std::to_string(need) +
+                ")");
+      return 1;
+    }
+    part = static_cast<C*>(ws);
+  }
+  int kchunk = (k + ns - 1) / ns;
+  kchunk = ((kchunk + BK - 1) / BK) * BK;
+  if (kchunk < BK) kchunk = BK;
+  ns = (k + kchunk - 1) / kchunk;
+  if (ns < 1) ns = 1;
+  if ((long long)batch * ns > 65535) {
+    set_error("zgemm: batch x split exceeds the grid limit");
+    return 1;
+  }
+  const dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM, batch * ns);
+  if (op == 0)
+    hipLaunchKernelGGL((k_zgemm<R, 0>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
+                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, ns > 1 ? part : nullptr);
+  else
+    hipLaunchKernelGGL((k_zgemm<R, 1>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
+                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, ns > 1 ? part : nullptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string("zgemm launch: ") + hipGetErrorString(e));
+    return 1;
+  }
+  if (ns > 1) {
+    const long long mn = (long long)m * n;
+    hipLaunchKernelGGL((k_splitk_reduce<R>), dim3((unsigned)((mn + 255) / 256), batch), dim3(256), 0, s, m, n, ns,
+                       (const C*)part, (C*)c, ldc, sc, (R)alpha, (R)beta);
+    e = hipGetLastError();
+    if (e != hipSuccess) {
+      set_error(std::string("zgemm reduce launch: ") + hipGetErrorString(e));
+      return 1;
+    }
+  }
+  return 0;
+}
+
+int launch_zgemm(int prec, int op, int m, int n, int k, double alpha, const void* a, long long lda, long long sa,
+                 const void* b, long long ldb, long long sb, double beta, void* c, long long ldc, long long sc,
+                 int batch, void* ws, size_t ws_bytes, hipStream_t s) {
+  return prec == kFP64 ? zgemm_t<double>(op, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc, batch, ws,
+                                         ws_bytes, s)
+                       : zgemm_t<float>(op, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc, batch, ws,
+                                        ws_bytes, s);
+}
+
+}  // namespace amt

@@ -1,0 +1,75 @@
+// Host-side launcher declarations shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "fft_lds.hpp"
+
+namespace amt {
+
+enum Precision { kFP32 = 0, kFP64 = 1 };
+
+// Butterflies per thread held in registers during one Stockham pass (radix 4..11; radix 2
+// and 3 passes hold twice as many). FP64 holds half as many to bound register use.
+constexpr int kTptF32 = 4;
+constexpr int kTptF64 = 2;
+
+// Factorise n into the supported radices; false if n is not 2,3,5,7,11-smooth or too long.
+bool make_axis_plan(int n, AxisPlan* out);
+// Threads a workgroup needs so every pass of `ap` over `nseq` sequences fits TPT per thread.
+int threads_needed(const AxisPlan& ap, int nseq, int tpt);
+// Twiddle table exp(-2 pi i m / n), m = 0..n-1, in double and float.
+void twiddles(int n, std::vector<double>& tw);
+
+// ------------------------------------------------------------------ plan (device state)
+struct Plan {
+  int n1 = 0, n2 = 0, n3 = 0;
+  int nk = 0;
+  int npw_pad = 0;
+  int smax = 0;  // sticks of the widest k-point
+  std::vector<int> npw;
+  std::vector<int> nsticks;
+  AxisPlan ap[3];
+  // device tables
+  int* d_flatmap = nullptr;    // [nk][npw_pad] flat C-order grid index, -1 on padding
+  int* d_stick_jk = nullptr;   // [nk][smax] j*n3 + k of each stick, -1 on padding
+  int* d_stick_map = nullptr;  // [nk][n1][smax] sphere index at (i, stick) or -1
+  int* d_nsticks = nullptr;    // [nk]
+  double* d_kin64 = nullptr;   // [nk][npw_pad] |k+G|^2/2 (0 on padding)
+  float* d_kin32 = nullptr;
+  double2* d_tw64[3] = {nullptr, nullptr, nullptr};
+  float2* d_tw32[3] = {nullptr, nullptr, nullptr};
+  // launch geometry
+  int stick_w[2] = {0, 0};         // sticks per workgroup, [prec]
+  int stick_threads[2] = {0, 0};   // workgroup size of the stick kernels
+  int plane_threads[2] = {0, 0};   // workgroup size of the plane kernel
+};
+
+void set_error(const std::string& msg);
+
+// Local part of H.psi (kinetic + V_eff) for nbk = nkb bands laid out [kb][npw_pad],
+// kb = k * nb + b for k in [k0, k0 + nkb / nb). `sticks` is nkb * n1 * smax complex.
+int launch_hpsi_local(const Plan& p, int prec, const void* psi, void* hpsi, int nb, int k0, int nkk,
+                      const void* veff, void* sticks, hipStream_t s);
+// sum over listed bands of w |u(r)|^2 into part[chunk][N] (double), then
+// rho = beta * rho + scale * sum_chunks part.
+int launch_density(const Plan& p, int prec, const void* psi, int nb, int k0, int nkk, const int* d_band_list,
+                   const double* d_band_weight, int nlist, int chunk, void* sticks, double* part, double* rho,
+                   double beta, double scale, hipStream_t s);
+int launch_scatter(const Plan& p, int prec, const void* psi, void* grid, int nb, int k, hipStream_t s);
+int launch_gather(const Plan& p, int prec, const void* grid, void* out, int nb, int k, double scale,
+                  hipStream_t s);
+int launch_fft3d(int prec, int n1, int n2, int n3, int batch, void* data, int sign, double scale,
+                 hipStream_t s);
+
+// C = alpha * A op(B) + beta * C, complex, row-major; op = 0: B is K x N, op = 1: B is N x K
+// and conj(B)^T is used. Batched with element strides. `ws` holds split-K partials.
+int launch_zgemm(int prec, int op, int m, int n, int k, double alpha, const void* a, long long lda,
+                 long long sa, const void* b, long long ldb, long long sb, double beta, void* c, long long ldc,
+                 long long sc, int batch, void* ws, size_t ws_bytes, hipStream_t s);
+size_t zgemm_workspace_bytes(int prec, int op, int m, int n, int k, int batch);
+
+}  // namespace amt

@@ -1,0 +1,257 @@
+"""Block Davidson on the GPU with the call surface of ``amaterasu.scf.eigen``.
+
+Mirrors ``scf/eigen/__init__.py:31-46`` (``solve``), ``scf/eigen/davidson.py:48-111``
+(``davidson``, ``DavidsonResult``), ``scf/eigen/subspace.py:17-88`` and
+``scf/eigen/precond.py:27-46``. The block-sized work runs in libamaterasu_amd:
+residuals and the Teter-Payne-Allan step are fused kernels, every Gram/projected
+matrix and every rotation is an MFMA complex GEMM. The small dense factorisations
+(Cholesky, Hermitian eigensolve) stay in FP64 on the GPU (torch.linalg on ROCm).
+
+One step differs in method, not in result: ``orthogonalize_against`` orthonormalises
+the projected block through the eigen-decomposition of its Gram matrix followed by a
+Cholesky re-orthonormalisation (CholeskyQR2), where the reference takes a thin SVD of
+the (rows x npw) block (``subspace.py:56-60``). Both keep exactly the directions whose
+singular value exceeds ``threshold * s_max`` and return an orthonormal basis of their
+span; the Gram route avoids a tall-skinny SVD on the device.
+
+``apply_hamiltonian`` may be the reference's closure over a :class:`Hamiltonian` of this
+package (device tensors flow through) or any numpy callable (blocks go through the host).
+numpy in gives numpy out.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import ops
+from ._lib import FP64
+
+_IMPLEMENTED = ("davidson",)
+
+
+@dataclass
+class DavidsonResult:
+    """``davidson.py:33-45``."""
+
+    eigenvalues: object
+    coefficients: object
+    residual_norms: object
+    iterations: int
+    converged: bool
+
+    @property
+    def nbands(self) -> int:
+        return len(self.eigenvalues)
+
+
+def available_eigensolvers() -> tuple:
+    return _IMPLEMENTED
+
+
+def _to_device(x, dtype=torch.complex128) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return x.to(device="cuda", dtype=dtype).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(x), dtype=np.complex128)).to("cuda", dtype)
+
+
+def _gram(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """a @ b^H on the matrix cores."""
+    if a.shape[0] == 0 or b.shape[0] == 0:
+        return torch.zeros((a.shape[0], b.shape[0]), dtype=a.dtype, device=a.device)
+    return ops.zgemm(a.contiguous(), b.contiguous(), 1)
+
+
+def _rotate(rot: torch.Tensor, block: torch.Tensor) -> torch.Tensor:
+    """rot @ block on the matrix cores."""
+    return ops.zgemm(rot.to(block.dtype).contiguous(), block.contiguous(), 0)
+
+
+def _hermitian(m: torch.Tensor) -> torch.Tensor:
+    m = m.to(torch.complex128)
+    return 0.5 * (m + m.conj().T)
+
+
+def _cholesky_orthonormalize(block: torch.Tensor):
+    """L^{-1} block with L L^H = block block^H; None if the Cholesky fails."""
+    s = _hermitian(_gram(block, block))
+    lower, info = torch.linalg.cholesky_ex(s)
+    if int(info.item()) != 0:
+        return None
+    eye = torch.eye(s.shape[0], dtype=s.dtype, device=s.device)
+    linv = torch.linalg.solve_triangular(lower, eye, upper=False)
+    out = _rotate(linv, block)
+    if not bool(torch.isfinite(torch.view_as_real(out)).all()):
+        return None
+    return out
+
+
+def _gram_orthonormalize(block: torch.Tensor, threshold: float) -> torch.Tensor:
+    """Rows spanning the directions with singular value > threshold * s_max, orthonormal.
+
+    s^2 and the right singular vectors come from the eigen-decomposition of the Gram
+    matrix (descending, as an SVD orders them); one CholeskyQR pass restores full
+    orthonormality for the weakest kept directions.
+    """
+    g = _hermitian(_gram(block, block))
+    w, u = torch.linalg.eigh(g)
+    w = torch.flip(w, dims=[0])
+    u = torch.flip(u, dims=[1])
+    s = torch.sqrt(torch.clamp(w, min=0.0))
+    s0 = float(s[0].item()) if s.numel() else 0.0
+    keep = s > threshold * max(s0, 1e-300)
+    nkeep = int(keep.sum().item())
+    if nkeep == 0:
+        return block[:0]
+    rot = (u[:, :nkeep].conj().T / s[:nkeep, None]).to(block.dtype)
+    v = _rotate(rot, block)
+    again = _cholesky_orthonormalize(v)
+    return v if again is None else again
+
+
+def orthonormalize(block):
+    """``subspace.py:17-32``: Cholesky of block block^H, SVD-equivalent fallback."""
+    host = not isinstance(block, torch.Tensor)
+    b = _to_device(block) if host else block.contiguous()
+    out = _cholesky_orthonormalize(b)
+    if out is None:
+        out = _gram_orthonormalize(b, 1e-10)
+    return out.cpu().numpy() if host else out
+
+
+def orthogonalize_against(block, basis, *, threshold: float = 1e-7):
+    """``subspace.py:41-60``: two projections off ``basis``, drop rows with norm <= threshold,
+    orthonormalise keeping singular values > threshold * s_max."""
+    host = not isinstance(block, torch.Tensor)
+    b = _to_device(block) if host else block.contiguous().clone()
+    q = _to_device(basis, b.dtype) if host else basis.contiguous()
+    for _ in range(2):
+        if q.shape[0] and b.shape[0]:
+            c = _gram(b, q)  # b q^H
+            ops.zgemm(c, q, 0, alpha=-1.0, beta=1.0, out=b)
+    norms = torch.linalg.vector_norm(b, dim=1)
+    keep = torch.nonzero(norms > threshold).flatten()
+    b = b.index_select(0, keep).contiguous()
+    if b.shape[0] == 0:
+        return b.cpu().numpy() if host else b
+    out = _gram_orthonormalize(b, threshold)
+    return out.cpu().numpy() if host else out
+
+
+def rayleigh_ritz(basis, h_basis, nbands: int):
+    """``subspace.py:63-76``: projected = conj(V) HV^T, Hermitised, eigh, lowest nbands."""
+    host = not isinstance(basis, torch.Tensor)
+    v = _to_device(basis) if host else basis
+    hv = _to_device(h_basis, v.dtype) if host else h_basis
+    m = _gram(v, hv)  # m[i][j] = sum_G v_i hv_j^*  =  conj(projected[i][j])
+    proj = _hermitian(m.conj())
+    w, vec = torch.linalg.eigh(proj)
+    rot = vec[:, :nbands].T.contiguous()
+    values, vv, hvv = w[:nbands], _rotate(rot, v), _rotate(rot, hv)
+    if host:
+        return values.cpu().numpy(), vv.cpu().numpy(), hvv.cpu().numpy()
+    return values, vv, hvv
+
+
+def teter_payne_allan(residual, kinetic, reference):
+    """``precond.py:27-46`` elementwise (host or device, no normalisation)."""
+    if isinstance(residual, torch.Tensor):
+        ref = torch.clamp(torch.as_tensor(reference, dtype=torch.float64, device=residual.device).reshape(-1),
+                          min=1e-8)
+        kin = torch.as_tensor(kinetic, dtype=torch.float64, device=residual.device)
+        x = kin[None, :] / ref[:, None]
+        p = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x))
+        return residual * (p / (p + 16.0 * x**4)).to(residual.dtype)
+    ref = np.maximum(np.atleast_1d(np.asarray(reference, dtype=float)), 1e-8)
+    x = np.asarray(kinetic)[None, :] / ref[:, None]
+    p = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x))
+    return residual * (p / (p + 16.0 * x**4))
+
+
+def random_bands(npw: int, nbands: int, kinetic, seed: int = 0, *, device: bool = False):
+    """``subspace.py:79-88``: the same numpy default_rng draws (bit-identical start), then the
+    orthonormalisation on the GPU."""
+    rng = np.random.default_rng(seed)
+    block = rng.normal(size=(nbands, npw)) + 1j * rng.normal(size=(nbands, npw))
+    block = block / (1.0 + np.asarray(kinetic))
+    out = orthonormalize(_to_device(block))
+    return out if device else out.cpu().numpy()
+
+
+class _Operator:
+    """Calls ``apply_hamiltonian`` with device tensors when it accepts them, else via numpy."""
+
+    def __init__(self, fn, dtype):
+        self.fn = fn
+        self.dtype = dtype
+        self.host = None
+
+    def __call__(self, block: torch.Tensor) -> torch.Tensor:
+        if self.host is None:
+            try:
+                out = self.fn(block)
+            except (TypeError, RuntimeError, AttributeError):
+                out = None
+            if isinstance(out, torch.Tensor):
+                self.host = False
+                return out.to(self.dtype).contiguous()
+            self.host = True
+        if self.host:
+            res = self.fn(block.to(torch.complex128).cpu().numpy())
+            return _to_device(res, self.dtype)
+        return self.fn(block).to(self.dtype).contiguous()
+
+
+def davidson(apply_hamiltonian, guess, kinetic, *, tolerance: float = 1e-8, max_iterations: int = 40,
+             subspace_factor: int = 4, precision="fp64") -> DavidsonResult:
+    """Lowest eigenpairs of a Hermitian operator from a starting block (``davidson.py:48-111``)."""
+    host = not isinstance(guess, torch.Tensor)
+    cdt = torch.complex128 if precision in ("fp64", FP64, None) else torch.complex64
+    op = _Operator(apply_hamiltonian, cdt)
+    kin = torch.as_tensor(np.asarray(kinetic, dtype=float) if not isinstance(kinetic, torch.Tensor) else kinetic,
+                          dtype=torch.float64).to("cuda")
+    basis = orthonormalize(_to_device(guess, cdt))
+    nbands, npw = basis.shape
+    max_subspace = min(max(subspace_factor, 2) * nbands, npw)
+    h_basis = op(basis)
+    eps, vec, hvec = rayleigh_ritz(basis, h_basis, nbands)
+    norms = torch.full((nbands,), float("inf"), dtype=torch.float64, device="cuda")
+    iteration = 0
+    for step in range(1, max_iterations + 1):
+        iteration = step
+        res, norms = ops.residual(hvec, vec, eps)
+        active = torch.nonzero(norms > tolerance).flatten()
+        if active.numel() == 0:
+            break
+        corr = ops.precondition(res, vec, kin, active)
+        new = orthogonalize_against(corr, basis)
+        if new.shape[0] == 0:
+            break
+        basis = torch.cat([basis, new])
+        h_basis = torch.cat([h_basis, op(new)])
+        eps, vec, hvec = rayleigh_ritz(basis, h_basis, nbands)
+        if basis.shape[0] + nbands > max_subspace:
+            basis, h_basis = vec, hvec
+    converged = bool((norms <= tolerance).all().item())
+    if host:
+        return DavidsonResult(eps.cpu().numpy(), vec.to(torch.complex128).cpu().numpy(), norms.cpu().numpy(),
+                              iteration, converged)
+    return DavidsonResult(eps, vec, norms, iteration, converged)
+
+
+def solve(name, apply_hamiltonian, guess, kinetic, *, apply_overlap=None, **kwargs) -> DavidsonResult:
+    """``eigen/__init__.py:31-46``."""
+    if name != "davidson":
+        raise NotImplementedError(
+            f"eigensolver {name!r} is not implemented yet; use "
+            f"{' or '.join(repr(x) for x in _IMPLEMENTED)} (scf.eigensolver)"
+        )
+    if apply_overlap is not None:
+        raise NotImplementedError("the generalised (PAW) Davidson is not part of the HIP path")
+    return davidson(apply_hamiltonian, guess, kinetic, **kwargs)
+
+
+__all__ = ["DavidsonResult", "available_eigensolvers", "davidson", "orthogonalize_against", "orthonormalize",
+           "random_bands", "rayleigh_ritz", "solve", "teter_payne_allan"]

@@ -1,0 +1,136 @@
+"""Torch-level wrappers around the C ABI: tensors in, tensors out, on the current stream.
+
+Every function here requires CUDA (HIP) tensors of the right dtype and layout and
+raises otherwise; nothing falls back to a CPU implementation.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from ._lib import FP32, FP64, AmtError, check, lib
+
+_WORKSPACES: dict = {}
+
+
+def stream_handle() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def precision_of(t: torch.Tensor) -> int:
+    if t.dtype in (torch.complex128, torch.float64):
+        return FP64
+    if t.dtype in (torch.complex64, torch.float32):
+        return FP32
+    raise AmtError(f"unsupported dtype {t.dtype}")
+
+
+def complex_dtype(prec: int):
+    return torch.complex128 if prec == FP64 else torch.complex64
+
+
+def real_dtype(prec: int):
+    return torch.float64 if prec == FP64 else torch.float32
+
+
+def _require(t: torch.Tensor, name: str):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise AmtError(f"{name} must be a HIP device tensor")
+    if not t.is_contiguous():
+        raise AmtError(f"{name} must be contiguous")
+
+
+def workspace(nbytes: int, tag: str = "ws") -> torch.Tensor:
+    """A cached device byte buffer of at least ``nbytes`` (grown, never shrunk)."""
+    key = (tag, torch.cuda.current_device())
+    buf = _WORKSPACES.get(key)
+    if buf is None or buf.numel() < nbytes:
+        buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device="cuda")
+        _WORKSPACES[key] = buf
+    return buf
+
+
+def zgemm(a: torch.Tensor, b: torch.Tensor, op: int, *, alpha: float = 1.0, beta: float = 0.0,
+          out: torch.Tensor | None = None) -> torch.Tensor:
+    """``alpha * a @ op(b) + beta * out`` on the matrix cores.
+
+    ``op = 0``: b is (k, n). ``op = 1``: b is (n, k) and ``conj(b)^T`` is used.
+    2-D operands, or 3-D operands batched over the leading axis.
+    """
+    _require(a, "a")
+    _require(b, "b")
+    if a.dtype != b.dtype or a.dtype not in (torch.complex64, torch.complex128):
+        raise AmtError("zgemm operands must share a complex dtype")
+    batched = a.dim() == 3
+    a3 = a if batched else a.unsqueeze(0)
+    b3 = b if batched else b.unsqueeze(0)
+    nb, m, k = a3.shape
+    if op == 0:
+        kb, n = b3.shape[1], b3.shape[2]
+    else:
+        n, kb = b3.shape[1], b3.shape[2]
+    if kb != k or b3.shape[0] != nb:
+        raise AmtError(f"zgemm shape mismatch: a {tuple(a.shape)}, b {tuple(b.shape)}, op {op}")
+    if out is None:
+        if beta != 0.0:
+            raise AmtError("beta != 0 needs out")
+        out = torch.empty((nb, m, n) if batched else (m, n), dtype=a.dtype, device=a.device)
+    _require(out, "out")
+    c3 = out if batched else out.unsqueeze(0)
+    if tuple(c3.shape) != (nb, m, n):
+        raise AmtError(f"zgemm out has shape {tuple(out.shape)}, expected {(nb, m, n)}")
+    prec = precision_of(a)
+    wsb = lib().amt_zgemm_workspace(prec, op, m, n, k, nb)
+    ws = workspace(wsb, "gemm") if wsb else None
+    check(lib().amt_zgemm(prec, op, m, n, k, float(alpha), a3.data_ptr(), k, m * k, b3.data_ptr(),
+                          b3.shape[2], b3.shape[1] * b3.shape[2], float(beta), c3.data_ptr(), n, m * n, nb,
+                          ws.data_ptr() if ws is not None else None, int(wsb), stream_handle()), "zgemm")
+    return out
+
+
+def fft3d(x: torch.Tensor, sign: int, scale: float = 1.0, *, inplace: bool = False) -> torch.Tensor:
+    """Batched 3D transform over the last three axes (sign +1: e^{+iGr}, -1: e^{-iGr})."""
+    _require(x, "x")
+    y = x if inplace else x.clone()
+    n1, n2, n3 = y.shape[-3:]
+    batch = int(y.numel() // (n1 * n2 * n3))
+    check(lib().amt_fft3d(precision_of(y), n1, n2, n3, batch, y.data_ptr(), int(sign), float(scale),
+                          stream_handle()), "fft3d")
+    return y
+
+
+def residual(hv: torch.Tensor, v: torch.Tensor, eps: torch.Tensor):
+    """R = HV - eps V and ||R|| per row (float64)."""
+    _require(hv, "hv")
+    _require(v, "v")
+    if hv.shape != v.shape or hv.dtype != v.dtype or v.dim() != 2:
+        raise AmtError(f"residual: hv {tuple(hv.shape)} and v {tuple(v.shape)} must be equal 2-D blocks")
+    nrow, npw = v.shape
+    if eps.numel() != nrow:
+        raise AmtError("residual: one eigenvalue per row")
+    res = torch.empty_like(v)
+    norms = torch.empty(nrow, dtype=torch.float64, device=v.device)
+    e = eps.to(device=v.device, dtype=torch.float64).contiguous()
+    check(lib().amt_residual(precision_of(v), hv.data_ptr(), v.data_ptr(), e.data_ptr(), res.data_ptr(),
+                             norms.data_ptr(), nrow, npw, stream_handle()), "residual")
+    return res, norms
+
+
+def precondition(res: torch.Tensor, v: torch.Tensor, kinetic: torch.Tensor, rows: torch.Tensor) -> torch.Tensor:
+    """Teter-Payne-Allan preconditioned, normalised residuals of the listed rows."""
+    _require(res, "res")
+    _require(v, "v")
+    if res.shape != v.shape or res.dtype != v.dtype or v.dim() != 2:
+        raise AmtError("precondition: res and v must be equal 2-D blocks")
+    prec = precision_of(v)
+    kin = kinetic.to(device=v.device, dtype=torch.float64 if prec == FP64 else torch.float32).contiguous()
+    if kin.numel() != v.shape[1]:
+        raise AmtError("precondition: kinetic must have one entry per plane wave")
+    r = rows.to(device=v.device, dtype=torch.int32).contiguous()
+    if r.numel() and (int(r.min().item()) < 0 or int(r.max().item()) >= v.shape[0]):
+        raise AmtError("precondition: row index out of range")
+    nout = int(r.numel())
+    out = torch.empty((nout, v.shape[1]), dtype=v.dtype, device=v.device)
+    check(lib().amt_precondition(prec, res.data_ptr(), v.data_ptr(), kin.data_ptr(), r.data_ptr(),
+                                 out.data_ptr(), nout, v.shape[1], stream_handle()), "precondition")
+    return out

@@ -1,0 +1,219 @@
+"""Norm-conserving datasets and Kleinman-Bylander projectors, assembled on the GPU.
+
+``read_psp8`` parses the ABINIT psp8 format the reference reads in
+``pseudo/readers/psp8.py:90-220`` (projectors stored as r*beta, densities as 4 pi rho,
+multiplied by r^2 on load). ``build_projectors`` produces the matrix of
+``pseudo/kb.py:260-343``:
+
+    <k+G|beta_{a,l,n,m}> = (4 pi / sqrt(Omega)) (-i)^l Y_lm(k+G) f_ln(|k+G|) exp(-i (k+G).tau_a)
+
+with rows atom-major, then radial channel, then m = -l..l (real Y_lm, l = 1 ordered
+y, z, x) and D block-diagonal per atom. The radial transform f_l(q) = int r^2 beta j_l(qr) dr
+is evaluated once per distinct |k+G| on the host with the same Simpson quadrature as
+the reference (``pseudo/base.py:130-144``); the (nproj x npw) matrix -- 0.7 GB in FP64
+for 64 silicon atoms -- is assembled directly in device memory.
+"""
+
+from __future__ import annotations
+
+import gzip
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+import torch
+from scipy.integrate import simpson
+from scipy.special import spherical_jn
+
+MAX_L = 3
+_XC_NAMES = {2: "LDA-PZ", 7: "LDA-PW", 11: "PBE", -116133: "PBEsol", -101130: "PBE", -1012: "LDA-PW"}
+
+
+@dataclass(frozen=True)
+class Channel:
+    l: int
+    index: int
+    energy: float
+    values: np.ndarray  # r beta_l(r)
+
+
+@dataclass
+class Dataset:
+    """What the operator needs from a norm-conserving pseudopotential file."""
+
+    symbol: str
+    atomic_number: float
+    valence_charge: float
+    r: np.ndarray
+    local_potential: np.ndarray
+    projectors: list
+    xc_functional: str | None = None
+    core_density: np.ndarray | None = None  # 4 pi r^2 rho_core
+    valence_density: np.ndarray | None = None  # 4 pi r^2 rho_valence
+    dij: np.ndarray | None = None
+    extra: dict = field(default_factory=dict)
+
+    def integrate(self, values) -> np.ndarray:
+        return simpson(np.asarray(values, dtype=float), x=self.r, axis=-1)
+
+
+def _num(tok: str) -> float:
+    return float(tok.replace("D", "E").replace("d", "e"))
+
+
+def read_psp8(path) -> Dataset:
+    path = Path(path)
+    opener = gzip.open if path.suffix == ".gz" else open
+    with opener(path, "rt", encoding="utf-8", errors="replace") as fh:
+        lines = fh.read().splitlines()
+    if len(lines) < 7:
+        raise ValueError(f"{path.name}: too short to be a psp8 file")
+    z, zion = _num(lines[1].split()[0]), _num(lines[1].split()[1])
+    code, ixc, lmax, lloc, mmax = (int(t) for t in lines[2].split()[:5])
+    if code != 8:
+        raise ValueError(f"{path.name}: pspcod {code}, not 8")
+    fchrg = _num(lines[3].split()[1])
+    counts = [int(t) for t in lines[4].split()[:5]]
+    switch = int(lines[5].split()[0])
+    if switch in (2, 3):
+        raise NotImplementedError(f"{path.name}: spin-orbit psp8 files are not supported")
+
+    def table(start):
+        rows = [[_num(t) for t in lines[start + i].split()[1:]] for i in range(mmax)]
+        return np.asarray(rows)
+
+    row = 6
+    chans = []
+    for l in range(lmax + 1):
+        if l == lloc or l >= len(counts) or counts[l] == 0:
+            continue
+        head = lines[row].split()
+        if int(head[0]) != l:
+            raise ValueError(f"{path.name}: expected l={l} at line {row + 1}")
+        ekb = [_num(t) for t in head[1 : 1 + counts[l]]]
+        tab = table(row + 1)
+        chans += [Channel(l, j, ekb[j], np.ascontiguousarray(tab[:, 1 + j])) for j in range(counts[l])]
+        row += 1 + mmax
+    if int(lines[row].split()[0]) != lloc:
+        raise ValueError(f"{path.name}: expected the local block at line {row + 1}")
+    tab = table(row + 1)
+    r, vloc = np.ascontiguousarray(tab[:, 0]), np.ascontiguousarray(tab[:, 1])
+    row += 1 + mmax
+    core = val = None
+    if fchrg > 0:
+        core = table(row)[:, 1] * r**2
+        row += mmax
+    if switch in (1, 3):
+        val = table(row)[:, 1] * r**2
+        row += mmax
+    return Dataset(symbol=lines[0].split()[0], atomic_number=z, valence_charge=zion, r=r, local_potential=vloc,
+                   projectors=chans, xc_functional=_XC_NAMES.get(ixc), core_density=core, valence_density=val,
+                   extra={"pspxc": ixc, "fchrg": fchrg})
+
+
+def _ylm(l: int, v: torch.Tensor) -> torch.Tensor:
+    """Real Y_lm, m = -l..l, of directions v (n, 3); the zero vector maps to +z."""
+    n = torch.linalg.vector_norm(v, dim=1)
+    u = torch.zeros_like(v)
+    u[:, 2] = 1.0
+    nz = n > 1e-14
+    u[nz] = v[nz] / n[nz, None]
+    x, y, z = u[:, 0], u[:, 1], u[:, 2]
+    pi = np.pi
+    if l == 0:
+        return torch.full((1, v.shape[0]), 0.5 / np.sqrt(pi), dtype=v.dtype, device=v.device)
+    if l == 1:
+        return np.sqrt(0.75 / pi) * torch.stack([y, z, x])
+    if l == 2:
+        c = 0.5 * np.sqrt(15.0 / pi)
+        return torch.stack([c * x * y, c * y * z, 0.25 * np.sqrt(5.0 / pi) * (3.0 * z * z - 1.0), c * x * z,
+                            0.25 * np.sqrt(15.0 / pi) * (x * x - y * y)])
+    if l == 3:
+        return torch.stack([
+            0.25 * np.sqrt(17.5 / pi) * y * (3.0 * x * x - y * y),
+            0.5 * np.sqrt(105.0 / pi) * x * y * z,
+            0.25 * np.sqrt(10.5 / pi) * y * (5.0 * z * z - 1.0),
+            0.25 * np.sqrt(7.0 / pi) * z * (5.0 * z * z - 3.0),
+            0.25 * np.sqrt(10.5 / pi) * x * (5.0 * z * z - 1.0),
+            0.25 * np.sqrt(105.0 / pi) * z * (x * x - y * y),
+            0.25 * np.sqrt(17.5 / pi) * x * (x * x - 3.0 * y * y),
+        ])
+    raise NotImplementedError(f"real spherical harmonics are implemented up to l={MAX_L}")
+
+
+def radial_form_factor(ds: Dataset, channel: Channel, q: np.ndarray) -> np.ndarray:
+    """f_l(q) = int r^2 beta_l(r) j_l(q r) dr (Simpson on the dataset mesh)."""
+    q = np.atleast_1d(np.asarray(q, dtype=float))
+    return ds.integrate((ds.r * channel.values)[None, :] * spherical_jn(channel.l, q[:, None] * ds.r[None, :]))
+
+
+@dataclass
+class Projectors:
+    """``KBProjectors``-compatible: ``matrix`` (nproj, npw) complex, ``dij``, ``labels``.
+
+    ``matrix`` is a device tensor; ``host_matrix()`` copies it to numpy for host consumers
+    (forces and stress read ``.matrix`` directly in the reference, ``properties/forces.py:152-171``).
+    """
+
+    matrix: torch.Tensor
+    dij: np.ndarray
+    labels: list
+
+    def __len__(self) -> int:
+        return len(self.labels)
+
+    def host_matrix(self) -> np.ndarray:
+        return self.matrix.cpu().numpy()
+
+
+def build_projectors(sphere, positions, symbols, datasets: dict, volume: float, *, device="cuda") -> Projectors:
+    """The projector matrix and D for one k-point sphere (``pseudo/kb.py:260-343``)."""
+    g2 = np.asarray(sphere.g2, dtype=float)
+    q = np.sqrt(g2)
+    # distinct |k+G| (shells): the radial transform depends on |k+G| only
+    uq, inv = np.unique(q, return_inverse=True)
+    kg = torch.from_numpy(np.asarray(sphere.cartesian, dtype=float)).to(device)
+    inv_t = torch.from_numpy(inv.astype(np.int64)).to(device)
+    pref = 4.0 * np.pi / np.sqrt(volume)
+    radial, ylm = {}, {}
+    for sym, ds in datasets.items():
+        for ch in ds.projectors:
+            f = torch.from_numpy(radial_form_factor(ds, ch, uq)).to(device)
+            radial[(sym, ch.l, ch.index)] = f[inv_t]
+            if ch.l not in ylm:
+                ylm[ch.l] = _ylm(ch.l, kg)
+    rows, labels, blocks = [], [], []
+    pos = torch.from_numpy(np.asarray(positions, dtype=float)).to(device)
+    for atom, sym in enumerate(symbols):
+        ds = datasets[sym]
+        phase = torch.exp(-1j * (kg @ pos[atom]))
+        alab = []
+        for ci, ch in enumerate(ds.projectors):
+            coef = pref * (-1j) ** ch.l
+            base = radial[(sym, ch.l, ch.index)] * phase
+            for mi in range(2 * ch.l + 1):
+                rows.append(coef * ylm[ch.l][mi] * base)
+                alab.append((atom, ch.l, ci, mi - ch.l))
+        nb = len(alab)
+        blk = np.zeros((nb, nb))
+        for a, (_, la, ca, ma) in enumerate(alab):
+            for b, (_, lb, cb, mb) in enumerate(alab):
+                if la == lb and ma == mb:
+                    if ds.dij is not None:
+                        blk[a, b] = ds.dij[ca, cb]
+                    elif ca == cb:
+                        blk[a, b] = ds.projectors[ca].energy
+        labels += alab
+        blocks.append(blk)
+    if not rows:
+        return Projectors(torch.zeros((0, len(g2)), dtype=torch.complex128, device=device), np.zeros((0, 0)), [])
+    total = sum(len(b) for b in blocks)
+    dij = np.zeros((total, total))
+    off = 0
+    for b in blocks:
+        dij[off : off + len(b), off : off + len(b)] = b
+        off += len(b)
+    return Projectors(matrix=torch.stack(rows).contiguous(), dij=dij, labels=labels)
+
+
+__all__ = ["Channel", "Dataset", "Projectors", "build_projectors", "radial_form_factor", "read_psp8"]

@@ -1,0 +1,90 @@
+"""Seeded synthetic inputs for the H.psi workloads (plain numpy, reproducible).
+
+* ``si8_kbatch``: the kernel-level H.psi batch -- conventional 8-atom Si cell
+  (a = 5.431 A), ecut 20 Ha (40 Ry) on the grid the reference's rule gives (42^3),
+  16 seeded k-points whose plane-wave counts are spread by +-8 % through a per-k cutoff
+  scale (1 + d_k)^(2/3), d_k ~ U[-0.08, 0.08], 64 bands per k with coefficients
+  CN(0, 1) / (1 + |k+G|^2)^2, and V_loc(r) a seeded normal field smoothed by a periodic
+  Gaussian of sigma = 2 grid cells, scaled to 1 Ha RMS.
+* ``si64_gamma``: 2x2x2 conventional supercell (64 atoms) with seeded Gaussian
+  displacements (sigma = 0.05 A), ecut 20 Ha, Gamma only (complex), 160 bands, 84^3.
+
+Everything here is numpy; device objects are built from it by the callers.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+#: CODATA 2014 Bohr radius in Angstrom, derived as ase.units does (amaterasu/units.py:70).
+BOHR_IN_ANG = 0.5291772105638411
+ANG_IN_BOHR = 1.0 / BOHR_IN_ANG
+
+DIAMOND_BASIS = np.array([[0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0],
+                          [0.25, 0.25, 0.25], [0.25, 0.75, 0.75], [0.75, 0.25, 0.75], [0.75, 0.75, 0.25]])
+
+
+def si_conventional(reps: int = 1, a_ang: float = 5.431, sigma_ang: float = 0.0, seed: int = 0):
+    """Cell (rows, Bohr) and Cartesian positions (Bohr) of a reps^3 diamond supercell."""
+    a = a_ang * ANG_IN_BOHR
+    cell = reps * a * np.eye(3)
+    shifts = np.stack(np.meshgrid(*[np.arange(reps)] * 3, indexing="ij"), axis=-1).reshape(-1, 3)
+    frac = (DIAMOND_BASIS[None, :, :] + shifts[:, None, :]).reshape(-1, 3) / reps
+    pos = frac @ cell
+    if sigma_ang > 0:
+        rng = np.random.default_rng(seed)
+        pos = pos + rng.normal(scale=sigma_ang * ANG_IN_BOHR, size=pos.shape)
+    return cell, pos
+
+
+def smooth_potential(shape, seed: int, sigma_cells: float = 2.0, rms: float = 1.0) -> np.ndarray:
+    """Seeded normal field, periodic Gaussian filter of sigma grid cells, scaled to ``rms`` Ha."""
+    rng = np.random.default_rng(seed)
+    field = rng.normal(size=shape)
+    f = [np.fft.fftfreq(n) for n in shape]  # cycles per cell
+    m2 = f[0][:, None, None] ** 2 + f[1][None, :, None] ** 2 + f[2][None, None, :] ** 2
+    filt = np.exp(-2.0 * np.pi**2 * sigma_cells**2 * m2)
+    v = np.fft.ifftn(np.fft.fftn(field) * filt).real
+    v -= v.mean()
+    v *= rms / np.sqrt(np.mean(v**2))
+    return np.ascontiguousarray(v)
+
+
+def seeded_coefficients(g2: np.ndarray, nbands: int, seed: int) -> np.ndarray:
+    """CN(0,1) / (1 + |k+G|^2)^2, shape (nbands, npw)."""
+    rng = np.random.default_rng(seed)
+    c = (rng.normal(size=(nbands, len(g2))) + 1j * rng.normal(size=(nbands, len(g2)))) / np.sqrt(2.0)
+    return c / (1.0 + g2[None, :]) ** 2
+
+
+@dataclass
+class KBatchSpec:
+    cell: np.ndarray
+    positions: np.ndarray
+    symbols: list
+    kpoints: np.ndarray
+    ecuts: np.ndarray  # per-k wavefunction cutoff (Ha)
+    ecut_rho: float
+    nbands: int
+    seed: int
+
+
+def si8_kbatch(nk: int = 16, nbands: int = 64, ecut: float = 20.0, spread: float = 0.08, seed: int = 2024):
+    cell, pos = si_conventional(1)
+    rng = np.random.default_rng(seed)
+    kpts = rng.uniform(-0.5, 0.5, size=(nk, 3))
+    d = rng.uniform(-spread, spread, size=nk)
+    ecuts = ecut * (1.0 + d) ** (2.0 / 3.0)
+    return KBatchSpec(cell=cell, positions=pos, symbols=["Si"] * 8, kpoints=kpts, ecuts=ecuts,
+                      ecut_rho=4.0 * ecut, nbands=nbands, seed=seed)
+
+
+def si64_gamma(nbands: int = 160, ecut: float = 20.0, seed: int = 64):
+    cell, pos = si_conventional(2, sigma_ang=0.05, seed=seed)
+    return KBatchSpec(cell=cell, positions=pos, symbols=["Si"] * 64, kpoints=np.zeros((1, 3)),
+                      ecuts=np.array([ecut]), ecut_rho=4.0 * ecut, nbands=nbands, seed=seed)
+
+
+__all__ = ["KBatchSpec", "si8_kbatch", "si64_gamma", "si_conventional", "smooth_potential", "seeded_coefficients"]

@@ -1,0 +1,126 @@
+/*
+ * amaterasu_amd -- C ABI of the MI355X (gfx950) H.psi / Davidson / density engine.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and nonzero on failure; amt_last_error()
+ *     then holds a message (thread-local). Nothing throws across the ABI.
+ *   - Device pointers are plain HIP device pointers (e.g. torch tensor data_ptr()).
+ *     Complex data is interleaved (re, im): complex64 = float2, complex128 = double2.
+ *     `precision` is 0 for FP32/complex64 and 1 for FP64/complex128.
+ *   - Band blocks are row-major [band][npw_pad]; a k-batched block is
+ *     [k][band][npw_pad]. Grids are C-ordered, flat = (i*n2 + j)*n3 + k.
+ *   - `stream` is a hipStream_t (0 = the null stream); every call is asynchronous on it.
+ *   - Workspaces (sticks, density partials, GEMM split-K partials) are passed in by the
+ *     caller; compute entry points allocate nothing. Only amt_plan_create allocates.
+ *
+ * Reference interfaces (amaterasu 0.1.0a2, paths relative to its source tree) that the
+ * entry points replace are cited per function. The Python binding a maintainer adds to
+ * the reference is in INTEGRATION.md.
+ */
+#ifndef AMATERASU_AMD_H
+#define AMATERASU_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct amt_plan amt_plan_t;
+
+/* Library identification and the last error of the calling thread. */
+const char* amt_version(void);
+const char* amt_last_error(void);
+/* One line describing the current HIP device (name, arch, CUs, LDS, clocks). */
+int amt_device_info(char* buf, int len);
+
+/*
+ * Device-resident tables for one FFT grid and a set of k-point spheres: the
+ * sphere->grid maps of FFTGrid.indices (basis/fft.py:174-190), the stick
+ * decomposition used by the pruned transforms, |k+G|^2/2 (basis/gvectors.py:100-103,
+ * GSphere.kinetic_energy) and the twiddle tables.
+ *   miller  : sum(npw) x 3 int32 Miller indices, k-point blocks concatenated in
+ *             GSphere order (basis/gvectors.py:153-155);
+ *   kinetic : sum(npw) float64 |k+G|^2/2 in the same order.
+ * Raises the reference's aliasing error (basis/fft.py:183-189) if 2 h_max + 1 > n.
+ */
+int amt_plan_create(amt_plan_t** out, int n1, int n2, int n3, int nk, const int* npw, int npw_pad,
+                    const int* miller, const double* kinetic);
+int amt_plan_destroy(amt_plan_t* plan);
+/* info[0..11] = n1, n2, n3, nk, npw_pad, smax, stick_w{fp32,fp64}, stick_threads{fp32,fp64},
+ * plane_threads{fp32,fp64} (0 = plane does not fit LDS at that precision). */
+int amt_plan_info(const amt_plan_t* plan, int* info, int len);
+int amt_plan_nsticks(const amt_plan_t* plan, int* out);
+/* Bytes of the stick workspace for `nbands` band rows (all k of one launch). */
+size_t amt_plan_stick_bytes(const amt_plan_t* plan, int precision, int nbands);
+
+/*
+ * Local part of H.psi for nb bands at each of the k-points k0..k0+nk-1:
+ *   hpsi = T_G psi + gather(FFT(V_eff . IFFT(scatter psi)))
+ * Replaces the first three lines of Hamiltonian.apply (hamiltonian/operator.py:60-66),
+ * i.e. apply_kinetic (hamiltonian/kinetic.py:22-24), FFTGrid.wave_to_real and
+ * FFTGrid.real_to_wave (basis/fft.py:237-243). V_eff is one spin channel, N reals of
+ * `precision`. psi/hpsi: [nk][nb][npw_pad]; padded columns of hpsi are not written.
+ * The non-local term is added afterwards with amt_zgemm / amt_kb_dij.
+ */
+int amt_hpsi_local(const amt_plan_t* plan, int precision, const void* psi, void* hpsi, int nb, int k0, int nk,
+                   const void* veff, void* sticks, void* stream);
+
+/*
+ * Density accumulation (scf/density.py:60-71): for each listed band row l
+ * (band_list[l] indexes rows of the [nk][nb][npw_pad] block),
+ *   rho = beta*rho + scale * sum_l band_weight[l] |u_l(r)|^2,  u = FFTGrid.wave_to_real.
+ * The caller passes scale = 1/Omega and weights (2/nspin) w_k f_nk, as density.py does.
+ * rho is float64 N; partials are float64 [ceil(nlist/chunk)][N].
+ */
+size_t amt_density_part_bytes(const amt_plan_t* plan, int nlist, int chunk);
+int amt_density(const amt_plan_t* plan, int precision, const void* psi, int nb, int k0, int nk,
+                const int* band_list, const double* band_weight, int nlist, int chunk, void* sticks,
+                double* part, double* rho, double beta, double scale, void* stream);
+
+/* FFTGrid.scatter (basis/fft.py:192-206): zero grid, place nb rows of sphere k. */
+int amt_scatter(const amt_plan_t* plan, int precision, const void* psi, void* grid, int nb, int k, void* stream);
+/* FFTGrid.gather (basis/fft.py:208-212) times `scale`; padded columns are zeroed. */
+int amt_gather(const amt_plan_t* plan, int precision, const void* grid, void* out, int nb, int k, double scale,
+               void* stream);
+/*
+ * In-place batched 3D transform over the last three axes, times `scale`:
+ *   sign = +1: u(r) = sum_G c_G e^{+iG.r}      (FFTGrid.to_real,      basis/fft.py:219-226, scale 1)
+ *   sign = -1: c_G = sum_r u(r) e^{-iG.r}      (FFTGrid.to_reciprocal, basis/fft.py:228-235, scale 1/N)
+ * Lengths must be 2,3,5,7,11-smooth (the grid rule of basis/fft.py:72-81).
+ */
+int amt_fft3d(int precision, int n1, int n2, int n3, int batch, void* data, int sign, double scale, void* stream);
+
+/*
+ * C = alpha A op(B) + beta C on the matrix cores (complex, row-major, real alpha/beta):
+ *   op = 0: B is k x n;  op = 1: B is n x k and conj(B)^T is used.
+ * Replaces the projector products of KBProjectors.project/apply (pseudo/kb.py:216-230),
+ * the Gram and projected matrices of orthonormalize / orthogonalize_against /
+ * rayleigh_ritz (scf/eigen/subspace.py:24, 53-54, 72) and their rotations (:75-76).
+ * Batched over `batch` with element strides sa/sb/sc. `ws` must hold
+ * amt_zgemm_workspace(...) bytes (0 when no split along k is used).
+ */
+size_t amt_zgemm_workspace(int precision, int op, int m, int n, int k, int batch);
+int amt_zgemm(int precision, int op, int m, int n, int k, double alpha, const void* a, long long lda, long long sa,
+              const void* b, long long ldb, long long sb, double beta, void* c, long long ldc, long long sc,
+              int batch, void* ws, size_t ws_bytes, void* stream);
+
+/* Davidson residual (scf/eigen/davidson.py:81-82): res = hv - eps v, norm = ||res|| per row (float64). */
+int amt_residual(int precision, const void* hv, const void* v, const double* eps, void* res, double* norm,
+                 int nrow, int npw, void* stream);
+/*
+ * Teter-Payne-Allan preconditioning and normalisation of the active residuals
+ * (scf/eigen/davidson.py:86-92, scf/eigen/precond.py:27-46): out row a is built from
+ * res/v row rows[a]. kinetic has the precision of the data.
+ */
+int amt_precondition(int precision, const void* res, const void* v, const void* kinetic, const int* rows,
+                     void* out, int nout, int npw, void* stream);
+/* (P D) of KBProjectors.apply (pseudo/kb.py:230) for block-diagonal D (real, nproj x nproj). */
+int amt_kb_dij(int precision, const void* proj, const void* dij, const int* blk_start, const int* blk_len,
+               void* out, int nrow, int nproj, long long sp, long long sd, long long so, int batch, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMATERASU_AMD_H */

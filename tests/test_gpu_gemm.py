@@ -1,0 +1,60 @@
+"""The MFMA complex GEMM (NN and NC, FP64 and FP32, split-K, batched) against numpy."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _c(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+
+
+@pytest.mark.parametrize("dtype,tol", [("complex128", 1e-13), ("complex64", 2e-5)])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 9), (64, 64, 16), (100, 70, 300), (8, 36, 754),
+                                   (160, 1152, 3000), (37, 5, 40000)])
+def test_zgemm_nn_and_nc(dtype, tol, m, n, k):
+    import torch
+
+    from amaterasu_amd import ops
+
+    a, b_nn, b_nc = _c((m, k), 1), _c((k, n), 2), _c((n, k), 3)
+    ta = torch.from_numpy(a).to("cuda", getattr(torch, dtype))
+    nn = ops.zgemm(ta, torch.from_numpy(b_nn).to("cuda", ta.dtype), 0).cpu().numpy()
+    nc = ops.zgemm(ta, torch.from_numpy(b_nc).to("cuda", ta.dtype), 1).cpu().numpy()
+    ref_nn, ref_nc = a @ b_nn, a @ b_nc.conj().T
+    scale = np.sqrt(k) * 2.0
+    assert np.abs(nn - ref_nn).max() / scale < tol
+    assert np.abs(nc - ref_nc).max() / scale < tol
+
+
+def test_zgemm_alpha_beta_and_batch():
+    import torch
+
+    from amaterasu_amd import ops
+
+    a, b, c = _c((3, 20, 50), 4), _c((3, 50, 30), 5), _c((3, 20, 30), 6)
+    tc = torch.from_numpy(c).cuda()
+    ops.zgemm(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), 0, alpha=-1.0, beta=1.0, out=tc)
+    assert np.abs(tc.cpu().numpy() - (c - a @ b)).max() < 1e-12
+    bc = _c((3, 30, 50), 7)
+    out = ops.zgemm(torch.from_numpy(a).cuda(), torch.from_numpy(bc).cuda(), 1).cpu().numpy()
+    assert np.abs(out - a @ bc.conj().transpose(0, 2, 1)).max() < 1e-12
+
+
+def test_mfma_lane_layout_single_tile():
+    """A 16x16x4 product with distinct small integers: exact in both precisions, so any
+    lane-layout mistake (transposed fragment, wrong row of the accumulator) shows up as
+    a wrong integer."""
+    import torch
+
+    from amaterasu_amd import ops
+
+    a = (np.arange(16 * 4).reshape(16, 4) % 7 + 1j * (np.arange(64).reshape(16, 4) % 5)).astype(np.complex128)
+    b = (np.arange(4 * 16).reshape(4, 16) % 3 - 1j * (np.arange(64).reshape(4, 16) % 4)).astype(np.complex128)
+    for dt in (torch.complex128, torch.complex64):
+        got = ops.zgemm(torch.from_numpy(a).to("cuda", dt), torch.from_numpy(b).to("cuda", dt), 0).cpu().numpy()
+        assert np.array_equal(got.astype(np.complex128), a @ b)
