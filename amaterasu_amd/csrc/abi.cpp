@@ -305,6 +305,25 @@ int amt_hpsi_local(const amt_plan_t* h, int precision, const void* psi, void* hp
   return launch_hpsi_local(p, precision, psi, hpsi, nb, k0, nkk, veff, sticks, (hipStream_t)stream);
 }
 
+int amt_hpsi_stage(const amt_plan_t* h, int precision, int stage, const void* psi, void* hpsi, int nb, int k0,
+                   int nkk, const void* veff, void* sticks, void* stream) {
+  if (stage < 1 || stage > 3) {
+    set_error("hpsi_stage: stage must be 1 (stick inverse), 2 (plane) or 3 (stick forward)");
+    return 1;
+  }
+  if (!h || !psi || !hpsi || !veff || !sticks) {
+    set_error("hpsi_stage: null argument");
+    return 1;
+  }
+  const Plan& p = h->p;
+  if (check_plane(p, precision)) return 1;
+  if (nb <= 0 || nkk <= 0 || k0 < 0 || k0 + nkk > p.nk || (long long)nb * nkk > 65535) {
+    set_error("hpsi_stage: band/k range outside the plan");
+    return 1;
+  }
+  return launch_hpsi_local(p, precision, psi, hpsi, nb, k0, nkk, veff, sticks, (hipStream_t)stream, stage);
+}
+
 size_t amt_density_part_bytes(const amt_plan_t* h, int nlist, int chunk) {
   if (!h || nlist <= 0 || chunk <= 0) return 0;
   const long long nch = (nlist + chunk - 1) / chunk;

@@ -317,7 +317,7 @@ static int check(hipError_t e, const char* what) {
 
 template <typename R>
 static int hpsi_local_t(const Plan& p, const void* psi_, void* hpsi_, int nb, int k0, int nkk, const void* veff_,
-                        void* sticks_, hipStream_t s) {
+                        void* sticks_, hipStream_t s, int stage) {
   using C = typename Cplx<R>::T;
   constexpr int prec = sizeof(R) == 8 ? kFP64 : kFP32;
   constexpr int TPT = sizeof(R) == 8 ? kTptF64 : kTptF32;
@@ -334,22 +334,30 @@ static int hpsi_local_t(const Plan& p, const void* psi_, void* hpsi_, int nb, in
   const size_t slds = (size_t)p.n1 * W * sizeof(C);
   const size_t plds = (size_t)p.n2 * p.n3 * sizeof(C);
   const R scale = (R)(1.0 / ((double)p.n1 * p.n2 * p.n3));
-  hipLaunchKernelGGL((k_stick_inverse<TPT, C>), sgrid, dim3(p.stick_threads[prec]), slds, s, psi, sticks,
-                     p.d_stick_map, p.n1, p.smax, p.npw_pad, nb, k0, p.ap[0], tw[0], W);
-  if (check(hipGetLastError(), "stick_inverse launch")) return 1;
-  hipLaunchKernelGGL((k_plane_vloc<TPT, C, R>), dim3(p.n1, nkb), dim3(p.plane_threads[prec]), plds, s, sticks,
-                     p.d_stick_jk, p.d_nsticks, veff, p.n1, p.n2, p.n3, p.smax, nb, k0, p.ap[1], p.ap[2], tw[1],
-                     tw[2]);
-  if (check(hipGetLastError(), "plane_vloc launch")) return 1;
-  hipLaunchKernelGGL((k_stick_forward<TPT, C, R>), sgrid, dim3(p.stick_threads[prec]), slds, s, sticks, psi,
-                     hpsi, p.d_stick_map, kin, p.n1, p.smax, p.npw_pad, nb, k0, p.ap[0], tw[0], W, scale);
-  return check(hipGetLastError(), "stick_forward launch");
+  // stage 0 runs all three kernels; 1, 2, 3 run one of them (for timing a single kernel)
+  if (stage == 0 || stage == 1) {
+    hipLaunchKernelGGL((k_stick_inverse<TPT, C>), sgrid, dim3(p.stick_threads[prec]), slds, s, psi, sticks,
+                       p.d_stick_map, p.n1, p.smax, p.npw_pad, nb, k0, p.ap[0], tw[0], W);
+    if (check(hipGetLastError(), "stick_inverse launch")) return 1;
+  }
+  if (stage == 0 || stage == 2) {
+    hipLaunchKernelGGL((k_plane_vloc<TPT, C, R>), dim3(p.n1, nkb), dim3(p.plane_threads[prec]), plds, s, sticks,
+                       p.d_stick_jk, p.d_nsticks, veff, p.n1, p.n2, p.n3, p.smax, nb, k0, p.ap[1], p.ap[2], tw[1],
+                       tw[2]);
+    if (check(hipGetLastError(), "plane_vloc launch")) return 1;
+  }
+  if (stage == 0 || stage == 3) {
+    hipLaunchKernelGGL((k_stick_forward<TPT, C, R>), sgrid, dim3(p.stick_threads[prec]), slds, s, sticks, psi,
+                       hpsi, p.d_stick_map, kin, p.n1, p.smax, p.npw_pad, nb, k0, p.ap[0], tw[0], W, scale);
+    if (check(hipGetLastError(), "stick_forward launch")) return 1;
+  }
+  return 0;
 }
 
 int launch_hpsi_local(const Plan& p, int prec, const void* psi, void* hpsi, int nb, int k0, int nkk,
-                      const void* veff, void* sticks, hipStream_t s) {
-  return prec == kFP64 ? hpsi_local_t<double>(p, psi, hpsi, nb, k0, nkk, veff, sticks, s)
-                       : hpsi_local_t<float>(p, psi, hpsi, nb, k0, nkk, veff, sticks, s);
+                      const void* veff, void* sticks, hipStream_t s, int stage) {
+  return prec == kFP64 ? hpsi_local_t<double>(p, psi, hpsi, nb, k0, nkk, veff, sticks, s, stage)
+                       : hpsi_local_t<float>(p, psi, hpsi, nb, k0, nkk, veff, sticks, s, stage);
 }
 
 template <typename R>

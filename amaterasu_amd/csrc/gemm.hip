@@ -9,8 +9,9 @@
 //   C_re += A_re B_re - A_im B_im,  C_im += A_re B_im + A_im B_re
 // i.e. four v_mfma_*_16x16x4 per 16x16x4 complex step (A_im is negated in registers).
 // The FP32 and FP64 instructions share one operand layout, so one template serves
-// both: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and receives
-// D[4 (l / 16) + q][l % 16], q = 0..3.
+// both on the input side: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]. The
+// accumulator differs (measured on gfx950): accumulator element q of lane l is
+// D[4 (l / 16) + q][l % 16] for FP32 and D[l / 16 + 4 q][l % 16] for FP64.
 //
 // Tiling: a 256-thread workgroup (4 wavefronts) computes a 64 x 64 complex tile; each
 // wavefront a 32 x 32 quarter as 2 x 2 MFMA blocks. K advances 16 at a time through
@@ -32,6 +33,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int q) { return (lane >> 4) * 4 + q; }
 };
 template <>
 struct Mfma<double> {
@@ -39,6 +41,7 @@ struct Mfma<double> {
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int q) { return (lane >> 4) + 4 * q; }
 };
 
 constexpr int BM = 64, BN = 64, BK = 16;
@@ -130,7 +133,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
     for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int row = m0 + wr + rb * 16 + (lane >> 4) * 4 + q;
+        const int row = m0 + wr + rb * 16 + Mfma<R>::row(lane, q);
         const int col = n0 + wc + cb * 16 + (lane & 15);
         if (row < M && col < N) {
           const C v = cmake<C>(accr[rb][cb][q], acci[rb][cb][q]);

@@ -53,7 +53,7 @@ void set_error(const std::string& msg);
 // Local part of H.psi (kinetic + V_eff) for nbk = nkb bands laid out [kb][npw_pad],
 // kb = k * nb + b for k in [k0, k0 + nkb / nb). `sticks` is nkb * n1 * smax complex.
 int launch_hpsi_local(const Plan& p, int prec, const void* psi, void* hpsi, int nb, int k0, int nkk,
-                      const void* veff, void* sticks, hipStream_t s);
+                      const void* veff, void* sticks, hipStream_t s, int stage = 0);
 // sum over listed bands of w |u(r)|^2 into part[chunk][N] (double), then
 // rho = beta * rho + scale * sum_chunks part.
 int launch_density(const Plan& p, int prec, const void* psi, int nb, int k0, int nkk, const int* d_band_list,

@@ -1,0 +1,256 @@
+#!/usr/bin/env python3
+"""H.psi throughput of the batched k-point workload on one MI355X.
+
+A step is one batched application of the Kohn-Sham Hamiltonian (kinetic + local
+V_eff through the pruned FFTs + Kleinman-Bylander projectors) to every band of every
+k-point of the Si8 kernel-batch workload: conventional 8-atom Si cell, ecut 20 Ha on a
+42^3 grid, 16 seeded k-points with plane-wave counts spread +-8 %, 64 bands each
+(1024 band-applications per step), inputs resident in HBM. ``value`` is
+band-applications per second in the mixed-precision (FP32) form; the FP64 form is
+reported alongside. The ``roofline`` object covers the dominant kernel (the per-plane
+2D FFT / V_eff kernel), timed with HIP events on the stream it runs on; the
+``cpu_baseline`` object times the FP64 numpy oracle of the same operator on a bounded
+sample of the same workload on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+METRIC = "Si64 SCF s-to-1e-6 Ha/atom & H·ψ bands/s: mixed+learned vs FP64 GPU"
+HBM_PEAK_GBS = 8000.0
+REPO = Path(__file__).resolve().parent
+
+
+def _traffic_from_profile():
+    """HBM bytes per launch of the plane kernel, from the committed rocprofv3 PMC summary."""
+    path = REPO / "profiles" / "plane_vloc_si8_kbatch_fp32_pmc.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch"):
+            try:
+                return float(line.split("=")[1])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def build_workload(precision: str):
+    import numpy as np
+    import torch
+
+    from amaterasu_amd.hamiltonian import Hamiltonian
+    from amaterasu_amd.system import build_system, load_dataset
+    from amaterasu_amd.workloads import seeded_coefficients, si8_kbatch, smooth_potential
+
+    spec = si8_kbatch()
+    system = build_system(spec.cell, spec.positions, spec.symbols, {"Si": load_dataset()}, spec.ecut_rho,
+                          spec.kpoints, spec.ecuts)
+    v = smooth_potential(system.grid.shape, seed=spec.seed)
+    ham = Hamiltonian(system, v, precision=precision)
+    plan = ham.plan
+    cdt = torch.complex64 if precision == "fp32" else torch.complex128
+    psi = torch.zeros((plan.nk, spec.nbands, plan.npw_pad), dtype=cdt, device="cuda")
+    for k, s in enumerate(system.wave_spheres):
+        psi[k, :, : len(s.g2)] = torch.from_numpy(seeded_coefficients(s.g2, spec.nbands, seed=1000 + k)).to(
+            "cuda", cdt)
+    return spec, system, v, ham, psi
+
+
+def time_steps(ham, psi, steps: int, warmup: int) -> float:
+    import torch
+
+    out = torch.zeros_like(psi)
+    for _ in range(warmup):
+        ham.plan.hpsi(psi, ham.veff(0, 0 if psi.dtype == torch.complex64 else 1), out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ham.plan.hpsi(psi, ham.veff(0, 0 if psi.dtype == torch.complex64 else 1), out=out)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def time_kernels(ham, psi, reps: int = 20):
+    """Average duration (ms) of each kernel of the step, HIP events on the launch stream."""
+    import torch
+
+    from amaterasu_amd import ops
+    from amaterasu_amd._lib import check, lib
+
+    plan = ham.plan
+    prec = ops.precision_of(psi)
+    nkk, nb, _ = psi.shape
+    out = torch.zeros_like(psi)
+    st = plan.sticks(prec, nkk * nb)
+    v = ham.veff(0, prec)
+    res = {}
+    names = {1: "stick_inverse", 2: "plane_vloc", 3: "stick_forward"}
+    def stage(i):
+        check(lib().amt_hpsi_stage(plan.handle, prec, i, psi.data_ptr(), out.data_ptr(), nb, 0, nkk, v.data_ptr(),
+                                   st.data_ptr(), ops.stream_handle()))
+
+    # The plane kernel works in place on the sticks, so stage 1 refills them (outside the
+    # events) before every timed launch.
+    for i in (1, 2, 3):
+        pairs = []
+        for _ in range(reps):
+            if i != 1:
+                stage(1)
+            if i == 3:
+                stage(2)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            stage(i)
+            e1.record()
+            pairs.append((e0, e1))
+        torch.cuda.synchronize()
+        res[names[i]] = sum(a.elapsed_time(b) for a, b in pairs) / reps
+    mat = plan.kb_tables(prec)[0]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        ops.zgemm(psi, mat, 1)
+    e1.record()
+    e1.synchronize()
+    res["kb_project_gemm"] = e0.elapsed_time(e1) / reps
+    proj = ops.zgemm(psi, mat, 1)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        ops.zgemm(proj, mat, 0, beta=1.0, out=out)
+    e1.record()
+    e1.synchronize()
+    res["kb_apply_gemm"] = e0.elapsed_time(e1) / reps
+    return res
+
+
+def cpu_baseline(spec, v, seconds: float = 15.0):
+    """FP64 numpy oracle of Hamiltonian.apply on a bounded sample of the same workload."""
+    import numpy as np
+
+    from amaterasu_amd.workloads import seeded_coefficients
+    from oracle.hamiltonian import Hamiltonian as OH
+    from oracle.pseudo import read_psp8
+    from oracle.system import build_system as obuild
+
+    nk_sample = 2
+    syso = obuild(spec.cell, spec.positions, spec.symbols,
+                  {"Si": read_psp8(REPO / "amaterasu_amd" / "data" / "Si.pbe.psp8.gz")}, float(spec.ecuts[0]),
+                  spec.ecut_rho, spec.kpoints[:nk_sample], wave_ecuts=spec.ecuts[:nk_sample], with_ewald=False)
+    ho = OH(syso, v[None])
+    blocks = [seeded_coefficients(s.g2, spec.nbands, seed=1000 + k) for k, s in enumerate(syso.wave_spheres)]
+    ho.apply(blocks[0][:2], 0)  # warm-up
+    done, t0 = 0, time.perf_counter()
+    while True:
+        for k in range(nk_sample):
+            ho.apply(blocks[k], k)
+            done += spec.nbands
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        cores = os.cpu_count()
+    return {
+        "value": done / dt,
+        "unit": "bands/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{done} band-applications of the FP64 numpy oracle (scipy.fft workers=1, OpenBLAS over "
+                  f"{cores} threads) on k-points 0..{nk_sample - 1} of the same workload, {dt:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the k-point/band sharding across "
+              "GPUs is described in DESIGN.md and not run here.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs an MI355X (HIP device)", file=sys.stderr)
+        return 1
+
+    spec, system, v, ham32, psi32 = build_workload("fp32")
+    nkb = psi32.shape[0] * psi32.shape[1]
+    t32 = time_steps(ham32, psi32, args.steps, args.warmup)
+    kern32 = time_kernels(ham32, psi32)
+
+    # FP64 form of the same step, and the FP32 error against it
+    from amaterasu_amd.hamiltonian import Hamiltonian
+
+    ham64 = Hamiltonian(system, v, precision="fp64")
+    psi64 = psi32.to(torch.complex128)
+    t64 = time_steps(ham64, psi64, max(5, args.steps // 2), 2)
+    h64 = ham64.plan.hpsi(psi64, ham64.veff(0, 1))
+    h32 = ham32.plan.hpsi(psi32, ham32.veff(0, 0)).to(torch.complex128)
+    rel32 = float((h32 - h64).abs().max().item() / h64.abs().max().item())
+
+    # roofline of the dominant kernel: the per-plane 2D FFT + V_eff kernel
+    plan = ham32.plan
+    n1, n2, n3 = plan.shape
+    nst = plan.nsticks()
+    nb = spec.nbands
+    s_c, s_r = 8, 4
+    alg_bytes = sum(nb * S * n1 * 2 * s_c for S in nst) + n1 * n2 * n3 * s_r
+    plane_ms = kern32["plane_vloc"]
+    achieved = alg_bytes / (plane_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic_from_profile(),
+                "kernel": "k_plane_vloc (fp32)", "algorithmic_bytes_per_launch": alg_bytes,
+                "avg_launch_ms": round(plane_ms, 5)}
+
+    line = {
+        "metric": METRIC,
+        "value": round(nkb / t32, 1),
+        "unit": "bands/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t32 * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "si8-kbatch: H.psi of Si8 cubic (a=5.431 A), ecut 20 Ha, grid 42^3, 16 seeded k-points "
+                        "(npw +-8 %), 64 bands each, nproj 144, V_loc seeded smooth field",
+            "bands_per_step": nkb,
+            "grid": list(plan.shape),
+            "npw": plan.npw,
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "fp64": {"value": round(nkb / t64, 1), "unit": "bands/s", "ms_per_step": round(t64 * 1e3, 4)},
+        "fp32_vs_fp64_max_rel_error": rel32,
+        "kernels_ms_fp32": {k: round(x, 5) for k, x in kern32.items()},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(spec, v, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

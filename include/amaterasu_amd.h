@@ -66,6 +66,11 @@ size_t amt_plan_stick_bytes(const amt_plan_t* plan, int precision, int nbands);
 int amt_hpsi_local(const amt_plan_t* plan, int precision, const void* psi, void* hpsi, int nb, int k0, int nk,
                    const void* veff, void* sticks, void* stream);
 
+/* One kernel of amt_hpsi_local (stage 1: stick inverse, 2: plane V_eff, 3: stick forward),
+ * for timing a single kernel; running 1, 2, 3 in order equals amt_hpsi_local. */
+int amt_hpsi_stage(const amt_plan_t* plan, int precision, int stage, const void* psi, void* hpsi, int nb, int k0,
+                   int nk, const void* veff, void* sticks, void* stream);
+
 /*
  * Density accumulation (scf/density.py:60-71): for each listed band row l
  * (band_list[l] indexes rows of the [nk][nb][npw_pad] block),
