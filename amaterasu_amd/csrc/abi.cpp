@@ -41,11 +41,11 @@ static int upload(T** dst, const std::vector<T>& src, const char* what) {
 }
 
 static void free_plan_device(Plan& p) {
-  void* ptrs[] = {p.d_flatmap, p.d_stick_jk, p.d_stick_map, p.d_nsticks, p.d_kin64, p.d_kin32,
-                  p.d_tw64[0],  p.d_tw64[1],  p.d_tw64[2],    p.d_tw32[0],  p.d_tw32[1], p.d_tw32[2]};
+  void* ptrs[] = {p.d_flatmap, p.d_stick_jk, p.d_stick_map, p.d_nsticks, p.d_rows,      p.d_nrows,  p.d_kin64,
+                  p.d_kin32,   p.d_tw64[0],  p.d_tw64[1],    p.d_tw64[2],  p.d_tw32[0], p.d_tw32[1], p.d_tw32[2]};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  p.d_flatmap = p.d_stick_jk = p.d_stick_map = p.d_nsticks = nullptr;
+  p.d_flatmap = p.d_stick_jk = p.d_stick_map = p.d_nsticks = p.d_rows = p.d_nrows = nullptr;
   p.d_kin64 = nullptr;
   p.d_kin32 = nullptr;
   for (int a = 0; a < 3; ++a) {
@@ -191,6 +191,15 @@ int amt_plan_create(amt_plan_t** out, int n1, int n2, int n3, int nk, const int*
       slot = g;
     }
   }
+  // rows j (plane index along axis 1) that hold at least one stick, per k
+  std::vector<int> rows((size_t)nk * n2, 0), nrows(nk, 0);
+  for (int k = 0; k < nk; ++k) {
+    std::vector<char> used(n2, 0);
+    for (int v : sticks[k]) used[v / n3] = 1;
+    for (int j = 0; j < n2; ++j)
+      if (used[j]) rows[(size_t)k * n2 + nrows[k]++] = j;
+  }
+  p.fast = (n1 == n2 && n2 == n3 && fast_length_supported(n1));
   std::vector<float> kin32(kin64.begin(), kin64.end());
   // launch geometry (must fit the LDS FFT's per-thread butterfly budget)
   const int lds_max = max_lds_bytes();
@@ -220,6 +229,7 @@ int amt_plan_create(amt_plan_t** out, int n1, int n2, int n3, int nk, const int*
   std::vector<int> nst = p.nsticks;
   if (!upload(&p.d_flatmap, flat, "plan flatmap") || !upload(&p.d_stick_jk, jk, "plan sticks") ||
       !upload(&p.d_stick_map, smap, "plan stick map") || !upload(&p.d_nsticks, nst, "plan nsticks") ||
+      !upload(&p.d_rows, rows, "plan rows") || !upload(&p.d_nrows, nrows, "plan nrows") ||
       !upload(&p.d_kin64, kin64, "plan kinetic") || !upload(&p.d_kin32, kin32, "plan kinetic32")) {
     free_plan_device(p);
     delete h;

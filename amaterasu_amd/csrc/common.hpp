@@ -24,31 +24,31 @@ struct Cplx<double> {
 };
 
 template <typename C>
-__device__ __forceinline__ C cmake(decltype(C::x) re, decltype(C::x) im) {
+__host__ __device__ __forceinline__ C cmake(decltype(C::x) re, decltype(C::x) im) {
   C c;
   c.x = re;
   c.y = im;
   return c;
 }
 template <typename C>
-__device__ __forceinline__ C cadd(C a, C b) {
+__host__ __device__ __forceinline__ C cadd(C a, C b) {
   return cmake<C>(a.x + b.x, a.y + b.y);
 }
 template <typename C>
-__device__ __forceinline__ C csub(C a, C b) {
+__host__ __device__ __forceinline__ C csub(C a, C b) {
   return cmake<C>(a.x - b.x, a.y - b.y);
 }
 template <typename C>
-__device__ __forceinline__ C cmul(C a, C b) {
+__host__ __device__ __forceinline__ C cmul(C a, C b) {
   return cmake<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 template <typename C>
-__device__ __forceinline__ C cscale(C a, decltype(C::x) s) {
+__host__ __device__ __forceinline__ C cscale(C a, decltype(C::x) s) {
   return cmake<C>(a.x * s, a.y * s);
 }
 // multiply by SIGN * i  (SIGN = +1 or -1)
 template <int SIGN, typename C>
-__device__ __forceinline__ C cmul_i(C a) {
+__host__ __device__ __forceinline__ C cmul_i(C a) {
   return SIGN > 0 ? cmake<C>(-a.y, a.x) : cmake<C>(a.y, -a.x);
 }
 

@@ -356,6 +356,13 @@ static int hpsi_local_t(const Plan& p, const void* psi_, void* hpsi_, int nb, in
 
 int launch_hpsi_local(const Plan& p, int prec, const void* psi, void* hpsi, int nb, int k0, int nkk,
                       const void* veff, void* sticks, hipStream_t s, int stage) {
+  if (p.fast) {
+    if (launch_hpsi_fast(p, prec, psi, hpsi, nb, k0, nkk, veff, sticks, s, stage)) {
+      set_error(std::string("hpsi (compile-time length) launch: ") + hipGetErrorString(hipGetLastError()));
+      return 1;
+    }
+    return 0;
+  }
   return prec == kFP64 ? hpsi_local_t<double>(p, psi, hpsi, nb, k0, nkk, veff, sticks, s, stage)
                        : hpsi_local_t<float>(p, psi, hpsi, nb, k0, nkk, veff, sticks, s, stage);
 }
@@ -380,7 +387,12 @@ static int density_t(const Plan& p, const void* psi_, int nb, int k0, int nkk, c
   const int W = p.stick_w[prec];
   const long long ngrid = (long long)p.n1 * p.n2 * p.n3;
   const int nchunk = nlist > 0 ? (nlist + chunk - 1) / chunk : 0;
-  if (nlist > 0) {
+  if (nlist > 0 && p.fast) {
+    if (launch_density_fast(p, prec, psi_, nb, k0, nkk, list, weight, nlist, chunk, sticks_, part, s)) {
+      set_error(std::string("density (compile-time length) launch: ") + hipGetErrorString(hipGetLastError()));
+      return 1;
+    }
+  } else if (nlist > 0) {
     hipLaunchKernelGGL((k_stick_inverse<TPT, C>), dim3((p.smax + W - 1) / W, nkb), dim3(p.stick_threads[prec]),
                        (size_t)p.n1 * W * sizeof(C), s, psi, sticks, p.d_stick_map, p.n1, p.smax, p.npw_pad, nb,
                        k0, p.ap[0], tw[0], W);

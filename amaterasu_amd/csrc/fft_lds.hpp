@@ -72,7 +72,7 @@ template <int R, int SIGN, typename C>
 __device__ __forceinline__ void dft(C* v);
 
 template <int R, int SIGN, typename C>
-__device__ __forceinline__ void dft_odd(C* v) {
+__host__ __device__ __forceinline__ void dft_odd(C* v) {
   using Rt = decltype(C::x);
   constexpr int H = (R - 1) / 2;
   C a[H], b[H];
@@ -107,14 +107,14 @@ __device__ __forceinline__ void dft_odd(C* v) {
 }
 
 template <int SIGN, typename C>
-__device__ __forceinline__ void dft2(C& a, C& b) {
+__host__ __device__ __forceinline__ void dft2(C& a, C& b) {
   C t = a;
   a = cadd(t, b);
   b = csub(t, b);
 }
 
 template <int SIGN, typename C>
-__device__ __forceinline__ void dft4v(C& v0, C& v1, C& v2, C& v3) {
+__host__ __device__ __forceinline__ void dft4v(C& v0, C& v1, C& v2, C& v3) {
   C t0 = cadd(v0, v2), t1 = csub(v0, v2);
   C t2 = cadd(v1, v3), t3 = cmul_i<SIGN>(csub(v1, v3));
   v0 = cadd(t0, t2);
@@ -133,12 +133,12 @@ template <>
 __device__ __forceinline__ void dft<2, -1, double2>(double2* v) { dft2<-1>(v[0], v[1]); }
 
 template <int SIGN, typename C>
-__device__ __forceinline__ void dft4(C* v) {
+__host__ __device__ __forceinline__ void dft4(C* v) {
   dft4v<SIGN>(v[0], v[1], v[2], v[3]);
 }
 
 template <int SIGN, typename C>
-__device__ __forceinline__ void dft8(C* v) {
+__host__ __device__ __forceinline__ void dft8(C* v) {
   using Rt = decltype(C::x);
   // even/odd split: X[q] = E[q] + w8^q O[q], X[q+4] = E[q] - w8^q O[q]
   C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
@@ -161,7 +161,7 @@ __device__ __forceinline__ void dft8(C* v) {
 }
 
 template <int R, int SIGN, typename C>
-__device__ __forceinline__ void dft_any(C* v) {
+__host__ __device__ __forceinline__ void dft_any(C* v) {
   if constexpr (R == 2) {
     dft2<SIGN>(v[0], v[1]);
   } else if constexpr (R == 4) {

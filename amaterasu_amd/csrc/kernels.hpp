@@ -38,6 +38,9 @@ struct Plan {
   int* d_stick_jk = nullptr;   // [nk][smax] j*n3 + k of each stick, -1 on padding
   int* d_stick_map = nullptr;  // [nk][n1][smax] sphere index at (i, stick) or -1
   int* d_nsticks = nullptr;    // [nk]
+  int* d_rows = nullptr;       // [nk][n2] the rows j that hold sticks (first nrows[k] entries)
+  int* d_nrows = nullptr;      // [nk]
+  bool fast = false;           // cubic grid of a length with compile-time kernels
   double* d_kin64 = nullptr;   // [nk][npw_pad] |k+G|^2/2 (0 on padding)
   float* d_kin32 = nullptr;
   double2* d_tw64[3] = {nullptr, nullptr, nullptr};
@@ -49,6 +52,13 @@ struct Plan {
 };
 
 void set_error(const std::string& msg);
+
+// Compile-time-length kernels (fft_fast.hip) for cubic grids of the listed lengths.
+bool fast_length_supported(int n);
+int launch_hpsi_fast(const Plan& p, int prec, const void* psi, void* hpsi, int nb, int k0, int nkk,
+                     const void* veff, void* sticks, hipStream_t s, int stage);
+int launch_density_fast(const Plan& p, int prec, const void* psi, int nb, int k0, int nkk, const int* list,
+                        const double* weight, int nlist, int chunk, void* sticks, double* part, hipStream_t s);
 
 // Local part of H.psi (kinetic + V_eff) for nbk = nkb bands laid out [kb][npw_pad],
 // kb = k * nb + b for k in [k0, k0 + nkb / nb). `sticks` is nkb * n1 * smax complex.

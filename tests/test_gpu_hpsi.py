@@ -33,6 +33,27 @@ def test_apply_matches_oracle_si2(si2_system):
         assert one.shape == c[2].shape and np.abs(one - ref[2]).max() / np.abs(ref).max() < 1e-12
 
 
+def test_apply_generic_grid_matches_oracle(si2_system):
+    """A non-cubic grid runs the generic (runtime-radix) kernels instead of the
+    compile-time-length ones; both must agree with the oracle."""
+    from amaterasu_amd.hamiltonian import Hamiltonian
+    from oracle.basis import FFTGrid
+    from oracle.hamiltonian import Hamiltonian as OH
+
+    s = copy.copy(si2_system)
+    s.grid = FFTGrid((24, 20, 22), si2_system.cell)
+    rng = np.random.default_rng(11)
+    v = rng.normal(size=(1, *s.grid.shape))
+    h, o = Hamiltonian(s, v), OH(s, v)
+    assert not h.plan.info[0] == h.plan.info[1] == h.plan.info[2]
+    for ik in range(2):
+        c = _block(len(s.wave_spheres[ik]), 3, 20 + ik)
+        got, ref = h.apply(c, ik), o.apply(c, ik)
+        assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12
+        got32 = Hamiltonian(s, v, precision="fp32").apply(c, ik)
+        assert np.abs(got32 - ref).max() / np.abs(ref).max() < 5e-6
+
+
 def test_apply_is_hermitian(si2_system):
     from amaterasu_amd.hamiltonian import Hamiltonian
 

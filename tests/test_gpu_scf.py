@@ -52,3 +52,22 @@ def test_mixed_precision_scf_error_is_stated(si2_system):
     r32 = run_scf(s, SCFSettings(tol_energy=1e-7, tol_density=1e-6, precision="mixed", max_steps=40))
     assert r32.converged
     assert abs(r32.energies.total - e64) / 2 < 1e-6
+
+
+def test_ionic_fields_match_oracle(si2_system):
+    """Device superposition of the local potential, the core and the starting valence density,
+    and the Ewald energy, against the oracle's system for the same Si2 cell."""
+    from amaterasu_amd.system import attach_ionic_fields, build_system, load_dataset
+
+    ds = load_dataset()
+    o = si2_system
+    s = build_system(o.cell, o.positions, o.symbols, {"Si": ds}, 32.0, o.kpoints.points, 8.0,
+                     weights=o.kpoints.weights, grid_shape=o.grid.shape)
+    attach_ionic_fields(s, {"Si": ds}, 32.0)
+    scale = np.abs(o.v_local).max()
+    assert np.abs(s.v_local.cpu().numpy() - o.v_local).max() / scale < 1e-12
+    if o.core_density is not None:
+        assert np.abs(s.core_density.cpu().numpy() - o.core_density).max() / np.abs(o.core_density).max() < 1e-12
+    rho0 = np.asarray(o.initial_density())
+    assert np.abs(s.initial_density().cpu().numpy().reshape(rho0.shape) - rho0).max() / rho0.max() < 1e-12
+    assert abs(s.ewald - o.ewald) < 1e-10
