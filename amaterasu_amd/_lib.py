@@ -15,6 +15,7 @@ FP32, FP64 = 0, 1
 
 _c_int, _c_dbl, _c_ll, _c_sz, _vp = ctypes.c_int, ctypes.c_double, ctypes.c_longlong, ctypes.c_size_t, ctypes.c_void_p
 _ip = ctypes.POINTER(ctypes.c_int)
+_c_float = ctypes.c_float
 _dp = ctypes.POINTER(ctypes.c_double)
 
 #: name -> (restype, argtypes); every symbol the header declares.
@@ -41,6 +42,12 @@ SIGNATURES = {
     "amt_residual": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp]),
     "amt_precondition": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp]),
     "amt_kb_dij": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_ll, _c_ll, _c_ll, _c_int, _vp]),
+    "amt_mlp_param_count": (_c_int, []),
+    "amt_mlp_features": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _ip, _c_dbl, _c_dbl, _vp, _vp]),
+    "amt_mlp_forward": (_c_int, [_vp, _c_ll, _vp, _vp, _vp, _vp, _vp]),
+    "amt_mlp_workspace": (_c_sz, [_c_ll]),
+    "amt_mlp_loss_grad": (_c_int, [_vp, _vp, _c_ll, _c_dbl, _vp, _vp, _vp, _vp, _vp, _vp, _c_sz, _vp]),
+    "amt_adam": (_c_int, [_vp, _vp, _vp, _vp, _c_int, _c_float, _c_float, _c_float, _c_float, _c_int, _vp]),
 }
 
 

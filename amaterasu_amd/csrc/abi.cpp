@@ -450,3 +450,82 @@ int amt_kb_dij(int precision, const void* proj, const void* dij, const int* blk_
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ learned start
+int amt_mlp_param_count(void) { return mlp_param_count(); }
+
+int amt_mlp_features(const double* cell, const double* frac, int natoms, int n1, int n2, int n3, const int* images,
+                     double rcut, double gamma, float* out, void* stream) {
+  if (!cell || !frac || !images || !out || natoms <= 0 || n1 <= 0 || n2 <= 0 || n3 <= 0) {
+    set_error("mlp_features: bad argument");
+    return 1;
+  }
+  for (int a = 0; a < 3; ++a)
+    if (images[a] < 0 || images[a] > 8) {
+      set_error("mlp_features: images per axis must be in 0..8");
+      return 1;
+    }
+  if (!(rcut > 0) || !(gamma > 0)) {
+    set_error("mlp_features: rcut and gamma must be positive");
+    return 1;
+  }
+  if (launch_mlp_features(cell, frac, natoms, n1, n2, n3, images[0], images[1], images[2], rcut, gamma, out,
+                          (hipStream_t)stream)) {
+    set_error(std::string("mlp_features launch: ") + hipGetErrorString(hipGetLastError()));
+    return 1;
+  }
+  return 0;
+}
+
+int amt_mlp_forward(const float* feats, long long npts, const float* theta, const float* mean, const float* std,
+                    float* y, void* stream) {
+  if (npts < 0 || (npts > 0 && (!feats || !theta || !mean || !std || !y))) {
+    set_error("mlp_forward: bad argument");
+    return 1;
+  }
+  if (launch_mlp_forward(feats, npts, theta, mean, std, y, (hipStream_t)stream)) {
+    set_error(std::string("mlp_forward launch: ") + hipGetErrorString(hipGetLastError()));
+    return 1;
+  }
+  return 0;
+}
+
+size_t amt_mlp_workspace(long long npts) {
+  const size_t nw = (size_t)mlp_train_waves(npts);
+  return nw * ((size_t)mlp_param_count() + 1) * sizeof(float);
+}
+
+int amt_mlp_loss_grad(const float* feats, const float* target, long long npts, double count, const float* theta,
+                      const float* mean, const float* std, float* grad, float* loss, void* ws, size_t ws_bytes,
+                      void* stream) {
+  if (npts <= 0 || !feats || !target || !theta || !mean || !std || !grad || !ws || !(count > 0)) {
+    set_error("mlp_loss_grad: bad argument");
+    return 1;
+  }
+  if (ws_bytes < amt_mlp_workspace(npts)) {
+    set_error("mlp_loss_grad: workspace smaller than amt_mlp_workspace()");
+    return 1;
+  }
+  const int nw = mlp_train_waves(npts);
+  float* part = static_cast<float*>(ws);
+  float* part_loss = part + (size_t)nw * mlp_param_count();
+  if (launch_mlp_train(feats, target, npts, count, theta, mean, std, part, part_loss, grad, loss,
+                       (hipStream_t)stream)) {
+    set_error(std::string("mlp_loss_grad launch: ") + hipGetErrorString(hipGetLastError()));
+    return 1;
+  }
+  return 0;
+}
+
+int amt_adam(float* theta, const float* grad, float* m, float* v, int n, float lr, float beta1, float beta2,
+             float eps, int step, void* stream) {
+  if (n <= 0 || !theta || !grad || !m || !v || step < 1) {
+    set_error("adam: bad argument");
+    return 1;
+  }
+  if (launch_adam(theta, grad, m, v, n, lr, beta1, beta2, eps, step, (hipStream_t)stream)) {
+    set_error(std::string("adam launch: ") + hipGetErrorString(hipGetLastError()));
+    return 1;
+  }
+  return 0;
+}

@@ -82,4 +82,17 @@ int launch_zgemm(int prec, int op, int m, int n, int k, double alpha, const void
                  long long sc, int batch, void* ws, size_t ws_bytes, hipStream_t s);
 size_t zgemm_workspace_bytes(int prec, int op, int m, int n, int k, int batch);
 
+// Learned starting density (mlp.hip).
+int mlp_param_count();
+int mlp_train_waves(long long npts);
+int launch_mlp_features(const double* cell, const double* frac, int natoms, int n1, int n2, int n3, int r1, int r2,
+                        int r3, double rcut, double gamma, float* out, hipStream_t s);
+int launch_mlp_forward(const float* feats, long long npts, const float* theta, const float* mean, const float* stdv,
+                       float* y, hipStream_t s);
+int launch_mlp_train(const float* feats, const float* target, long long npts, double count, const float* theta,
+                     const float* mean, const float* stdv, float* part, float* part_loss, float* grad, float* loss,
+                     hipStream_t s);
+int launch_adam(float* theta, const float* grad, float* m, float* v, int n, float lr, float beta1, float beta2,
+                float eps, int step, hipStream_t s);
+
 }  // namespace amt

@@ -8,6 +8,8 @@
   Gaussian of sigma = 2 grid cells, scaled to 1 Ha RMS.
 * ``si64_gamma``: 2x2x2 conventional supercell (64 atoms) with seeded Gaussian
   displacements (sigma = 0.05 A), ecut 20 Ha, Gamma only (complex), 160 bands, 84^3.
+* ``si64_scf_system``: the same cell as a device system ready for the SCF loop (PBE,
+  norm-conserving Si, local potential / core / starting density superposed on the device).
 
 Everything here is numpy; device objects are built from it by the callers.
 """
@@ -87,4 +89,15 @@ def si64_gamma(nbands: int = 160, ecut: float = 20.0, seed: int = 64):
                       ecuts=np.array([ecut]), ecut_rho=4.0 * ecut, nbands=nbands, seed=seed)
 
 
-__all__ = ["KBatchSpec", "si8_kbatch", "si64_gamma", "si_conventional", "smooth_potential", "seeded_coefficients"]
+def si64_scf_system(nbands: int = 160, ecut: float = 20.0, seed: int = 64, xc: str = "PBE"):
+    """Device system for the Si64 Gamma-point SCF (needs the GPU)."""
+    from .system import attach_ionic_fields, build_system, load_dataset
+
+    spec = si64_gamma(nbands=nbands, ecut=ecut, seed=seed)
+    ds = load_dataset()
+    system = build_system(spec.cell, spec.positions, spec.symbols, {"Si": ds}, spec.ecut_rho, spec.kpoints,
+                          spec.ecuts, nbands=spec.nbands)
+    return attach_ionic_fields(system, {"Si": ds}, spec.ecut_rho, xc=xc)
+
+
+__all__ = ["KBatchSpec", "si64_scf_system", "si8_kbatch", "si64_gamma", "si_conventional", "smooth_potential", "seeded_coefficients"]

@@ -124,6 +124,35 @@ int amt_precondition(int precision, const void* res, const void* v, const void* 
 int amt_kb_dij(int precision, const void* proj, const void* dij, const int* blk_start, const int* blk_len,
                void* out, int nrow, int nproj, long long sp, long long sd, long long so, int batch, void* stream);
 
+/*
+ * Learned starting density. No counterpart in the reference: it produces the density=
+ * argument of run_scf (scf/driver.py:267-299) as a correction over the atomic superposition
+ * of PlaneWaveSystem.initial_density (hamiltonian/system.py:147-190), which stays the
+ * fallback. Model: 16 radial features of the atoms within r_c of a grid point, three
+ * hidden layers of 64 SiLU units, one linear output; bf16 matrix-core operands, FP32 sums.
+ * Parameters are one FP32 vector of amt_mlp_param_count() entries:
+ *   W1[64][16] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4.
+ */
+int amt_mlp_param_count(void);
+/* Features of every point of an n1 x n2 x n3 grid (C order) into out[N][16] (float32).
+ * cell: 3x3 row vectors (Bohr); frac: natoms x 3 fractional coordinates (host pointers are
+ * not accepted: both live in device memory). images[3]: lattice images searched per axis. */
+int amt_mlp_features(const double* cell, const double* frac, int natoms, int n1, int n2, int n3,
+                     const int* images, double rcut, double gamma, float* out, void* stream);
+/* y[p] = model output at each of npts feature rows (mean/std: 16 floats each). */
+int amt_mlp_forward(const float* feats, long long npts, const float* theta, const float* mean,
+                    const float* std, float* y, void* stream);
+/* Bytes of workspace amt_mlp_loss_grad needs for npts points. */
+size_t amt_mlp_workspace(long long npts);
+/* grad = d/dtheta of (1/count) sum_p (y_p - target_p)^2 over the npts rows, loss = that mean
+ * (count = npts for a plain mean). Deterministic (fixed reduction order). */
+int amt_mlp_loss_grad(const float* feats, const float* target, long long npts, double count, const float* theta,
+                      const float* mean, const float* std, float* grad, float* loss, void* ws, size_t ws_bytes,
+                      void* stream);
+/* One Adam step (Kingma & Ba, bias-corrected; step counts from 1) on n parameters. */
+int amt_adam(float* theta, const float* grad, float* m, float* v, int n, float lr, float beta1, float beta2,
+             float eps, int step, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
