@@ -1,0 +1,198 @@
+"""Block Davidson over all k-points of a spin channel at once (``solve_batched``).
+
+Each k-point runs exactly the iteration of ``scf/eigen/davidson.py:48-111`` -- its own
+active set, its own dropped directions, its own subspace size and collapse, its own
+convergence -- but the k-points advance together so that every step is one launch per
+kernel over all of them: one H.psi over the padded block ``[nk][rows][npw_pad]``, one
+batched GEMM per Gram / projected matrix / rotation, one batched Cholesky and one
+batched Hermitian eigensolve.
+
+Ragged shapes are padded, never masked by branching on the device:
+* plane waves: every row is ``npw_pad`` long with zeros past the k-point's own count
+  (H.psi, the GEMMs and the preconditioner keep the padding zero);
+* subspace rows: k-point k uses the first m_k rows of a ``[nk][M][npw_pad]`` basis; the
+  unused rows are zero, and their diagonal of every projected (Gram) matrix is set to a
+  large value (to one), so the padded eigenpairs sort above every physical one (the
+  Cholesky of the padded block is the identity) and never mix with them;
+* new directions: k-point k contributes n_k <= nb rows of a zero-padded block.
+
+A k-point that has converged (or that stopped adding directions) is frozen: its
+eigenpairs and iteration count are those of the step at which the per-k loop would
+have returned.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import ops
+from .eigen import orthonormalize
+
+_PAD_EIG = 1.0e6
+
+
+@dataclass
+class BatchedResult:
+    """Per k-point ``DavidsonResult`` fields, stacked: eigenvalues [nk][nb], coefficients
+    [nk][nb][npw_pad] (zero past each k's npw), residual norms, iterations, converged."""
+
+    eigenvalues: torch.Tensor
+    coefficients: torch.Tensor
+    residual_norms: torch.Tensor
+    iterations: np.ndarray
+    converged: np.ndarray
+
+
+def _herm(m: torch.Tensor) -> torch.Tensor:
+    m = m.to(torch.complex128)
+    return 0.5 * (m + m.conj().transpose(-1, -2))
+
+
+def _bgram(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """a @ b^H per batch entry on the matrix cores."""
+    return ops.zgemm(a, b, 1)
+
+
+def _brot(rot: torch.Tensor, block: torch.Tensor) -> torch.Tensor:
+    return ops.zgemm(rot.to(block.dtype).contiguous(), block, 0)
+
+
+def _pad_diag(m: torch.Tensor, used: torch.Tensor, value: float) -> torch.Tensor:
+    """Set the diagonal of rows/cols >= used[k] to ``value`` (those rows/cols are zero)."""
+    n = m.shape[-1]
+    idx = torch.arange(n, device=m.device)
+    mask = idx[None, :] >= used[:, None]  # [nk][n]
+    diag = torch.diagonal(m, dim1=-2, dim2=-1)
+    diag += mask.to(m.dtype) * value
+    return m
+
+
+def _cholesky_rows(block: torch.Tensor, used: torch.Tensor):
+    """L^{-1} block per k (rows >= used[k] are zero and stay zero); None where it fails."""
+    s = _pad_diag(_herm(_bgram(block, block)), used, 1.0)
+    low, info = torch.linalg.cholesky_ex(s)
+    eye = torch.eye(s.shape[-1], dtype=s.dtype, device=s.device).expand_as(s)
+    linv = torch.linalg.solve_triangular(low, eye, upper=False)
+    return _brot(linv, block), info
+
+
+def _gram_rows(block: torch.Tensor, used: torch.Tensor, threshold: float):
+    """Per k: keep directions with singular value > threshold * s_max, orthonormal
+    (the Gram-matrix route of ``eigen.orthogonalize_against``). Returns (rows, kept)."""
+    g = _herm(_bgram(block, block))
+    w, u = torch.linalg.eigh(g)  # ascending; padded zero rows give zero eigenvalues
+    w = torch.flip(w, dims=[-1])
+    u = torch.flip(u, dims=[-1])
+    s = torch.sqrt(torch.clamp(w, min=0.0))
+    s0 = torch.clamp(s[:, :1], min=1e-300)
+    keep = (s > threshold * s0) & (torch.arange(s.shape[-1], device=s.device)[None, :] < used[:, None])
+    kept = keep.sum(-1)
+    inv = torch.where(keep, 1.0 / torch.where(keep, s, torch.ones_like(s)), torch.zeros_like(s))
+    rot = (u.conj().transpose(-1, -2) * inv[:, :, None]).to(block.dtype)  # rows beyond kept are zero
+    v = _brot(rot, block)
+    again, info = _cholesky_rows(v, kept)
+    ok = info == 0
+    if bool(ok.all()):
+        return again, kept
+    return torch.where(ok[:, None, None], again, v), kept
+
+
+def _rayleigh_ritz(basis, h_basis, used, nbands):
+    m = _bgram(basis, h_basis)  # conj of the projected matrix
+    proj = _pad_diag(_herm(m.conj()), used, _PAD_EIG)
+    w, vec = torch.linalg.eigh(proj)
+    rot = vec[:, :, :nbands].transpose(-1, -2).contiguous()
+    return w[:, :nbands].contiguous(), _brot(rot, basis), _brot(rot, h_basis)
+
+
+def solve_batched(apply_block, guess: torch.Tensor, kinetic: torch.Tensor, npw, *, tolerance: float = 1e-8,
+                  max_iterations: int = 40, subspace_factor: int = 4) -> BatchedResult:
+    """Lowest ``nb`` eigenpairs at every k-point of ``guess`` ``[nk][nb][npw_pad]``.
+
+    ``apply_block(x)`` returns H x for a zero-padded ``[nk][rows][npw_pad]`` block (the
+    device Hamiltonian's ``apply_batched``); ``kinetic`` is ``[nk][npw_pad]`` (zero past
+    each k's npw) in the precision of ``guess``; ``npw`` the per-k plane-wave counts.
+    """
+    nk, nb, npad = guess.shape
+    dev = guess.device
+    cdt = guess.dtype
+    maxsub = [min(max(subspace_factor, 2) * nb, int(n)) for n in npw]
+    cap = max(maxsub)
+    full = torch.full((nk,), nb, dtype=torch.long, device=dev)
+    v0, info = _cholesky_rows(guess.contiguous(), full)
+    for k in torch.nonzero(info).flatten().tolist():  # rank-deficient guess: the SVD-equivalent route
+        v0[k, :, : int(npw[k])] = orthonormalize(guess[k, :, : int(npw[k])].contiguous())
+    basis = torch.zeros((nk, cap, npad), dtype=cdt, device=dev)
+    hbasis = torch.zeros_like(basis)
+    basis[:, :nb] = v0
+    hbasis[:, :nb] = apply_block(v0)
+    m = np.full(nk, nb)
+    eps, vec, hvec = _rayleigh_ritz(basis[:, :nb], hbasis[:, :nb], full, nb)
+    # frozen outputs of the k-points that have returned
+    out_eps, out_vec = eps.clone(), vec.clone()
+    out_norm = torch.full((nk, nb), float("inf"), dtype=torch.float64, device=dev)
+    running = np.ones(nk, dtype=bool)
+    iters = np.zeros(nk, dtype=int)
+    kin = kinetic.to(dev).contiguous()
+    for step in range(1, max_iterations + 1):
+        iters[running] = step
+        res, norms = ops.residual(hvec.reshape(nk * nb, npad), vec.reshape(nk * nb, npad), eps.reshape(-1))
+        norms = norms.reshape(nk, nb)
+        out_norm[torch.as_tensor(running, device=dev)] = norms[torch.as_tensor(running, device=dev)]
+        active = (norms > tolerance).cpu().numpy() & running[:, None]
+        finished = running & ~active.any(axis=1)
+        running &= ~finished
+        if not running.any():
+            break
+        nnew = active.sum(axis=1)
+        width = int(nnew.max())
+        corr = torch.zeros((nk, width, npad), dtype=cdt, device=dev)
+        for k in np.flatnonzero(running):
+            rows = torch.as_tensor(np.flatnonzero(active[k]) + k * nb, dtype=torch.int32, device=dev)
+            corr[k, : nnew[k]] = ops.precondition(res, vec.reshape(nk * nb, npad), kin[k], rows)
+        # orthogonalise against the current basis (its padded rows are zero), twice
+        cur = int(m.max())
+        for _ in range(2):
+            c = _bgram(corr, basis[:, :cur])
+            ops.zgemm(c, basis[:, :cur], 0, alpha=-1.0, beta=1.0, out=corr)
+        rn = torch.linalg.vector_norm(corr, dim=-1)
+        keep = (rn > 1e-7).cpu().numpy() & (np.arange(width)[None, :] < nnew[:, None])
+        nkeep = keep.sum(axis=1)
+        # compact the kept rows to the front of each k's block
+        order = np.argsort(~keep, axis=1, kind="stable")
+        corr = torch.gather(corr, 1, torch.as_tensor(order, device=dev)[:, :, None].expand(-1, -1, npad))
+        corr = corr * torch.as_tensor(np.arange(width)[None, :] < nkeep[:, None], device=dev)[:, :, None]
+        new, kept_t = _gram_rows(corr.contiguous(), torch.as_tensor(nkeep, device=dev), 1e-7)
+        kept = kept_t.cpu().numpy()
+        stopped = running & (kept == 0)
+        running &= ~stopped
+        if not running.any():
+            break
+        kept = np.where(running, kept, 0)
+        hnew = apply_block(new.contiguous())
+        for k in np.flatnonzero(running):
+            basis[k, m[k] : m[k] + kept[k]] = new[k, : kept[k]]
+            hbasis[k, m[k] : m[k] + kept[k]] = hnew[k, : kept[k]]
+        m = m + kept
+        cur = int(m.max())
+        eps_n, vec_n, hvec_n = _rayleigh_ritz(basis[:, :cur], hbasis[:, :cur], torch.as_tensor(m, device=dev), nb)
+        run_t = torch.as_tensor(running, device=dev)
+        eps = torch.where(run_t[:, None], eps_n, eps)
+        vec = torch.where(run_t[:, None, None], vec_n, vec)
+        hvec = torch.where(run_t[:, None, None], hvec_n, hvec)
+        out_eps = torch.where(run_t[:, None], eps, out_eps)
+        out_vec = torch.where(run_t[:, None, None], vec, out_vec)
+        for k in np.flatnonzero(running & (m + nb > np.asarray(maxsub))):
+            basis[k].zero_()
+            hbasis[k].zero_()
+            basis[k, :nb] = vec[k]
+            hbasis[k, :nb] = hvec[k]
+            m[k] = nb
+    conv = (out_norm <= tolerance).all(dim=1).cpu().numpy()
+    return BatchedResult(out_eps, out_vec, out_norm, iters, conv)
+
+
+__all__ = ["BatchedResult", "solve_batched"]

@@ -218,8 +218,13 @@ int amt_plan_create(amt_plan_t** out, int n1, int n2, int n3, int nk, const int*
     pt = ((pt + 63) / 64) * 64;
     if (pt < 64) pt = 64;
     const long long plds = (long long)n2 * n3 * csz;
-    // 0 marks a precision whose plane does not fit one workgroup on this device
-    p.plane_threads[prec] = (pt <= 1024 && plds <= lds_max) ? pt : 0;
+    // 0 marks a precision whose plane does not fit one workgroup on this device. The
+    // compile-time-length kernels use a fixed 256-thread workgroup (512 for the density)
+    // and add the row list (n2 ints) to the plane.
+    if (p.fast)
+      p.plane_threads[prec] = (plds + 4LL * n2 <= lds_max) ? 256 : 0;
+    else
+      p.plane_threads[prec] = (pt <= 1024 && plds <= lds_max) ? pt : 0;
   }
   if (p.stick_threads[0] > 1024 || p.stick_threads[1] > 1024) {
     set_error("plan_create: axis 0 too long for the stick kernels");

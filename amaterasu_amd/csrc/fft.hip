@@ -117,7 +117,7 @@ __global__ void k_gather(const C* __restrict__ grid, C* __restrict__ out, const 
 template <int SIGN, int TPT, typename C>
 __global__ void k_axis_fft(C* __restrict__ data, AxisPlan ap, long long inner, long long nouter, int W,
                            const C* __restrict__ tw, decltype(C::x) scale) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int n = ap.n;
   const int tid = threadIdx.x, nth = blockDim.x;
@@ -154,7 +154,7 @@ template <int TPT, typename C>
 __global__ void k_stick_inverse(const C* __restrict__ psi, C* __restrict__ sticks,
                                 const int* __restrict__ stick_map, int n1, int smax, int npw_pad, int nb, int k0,
                                 AxisPlan ap1, const C* __restrict__ tw1, int W) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int tid = threadIdx.x, nth = blockDim.x;
   const int kb = blockIdx.y;
@@ -186,7 +186,7 @@ __global__ void k_plane_vloc(C* __restrict__ sticks, const int* __restrict__ sti
                              const int* __restrict__ nsticks, const R* __restrict__ veff, int n1, int n2, int n3,
                              int smax, int nb, int k0, AxisPlan ap2, AxisPlan ap3, const C* __restrict__ tw2,
                              const C* __restrict__ tw3) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   C* plane = reinterpret_cast<C*>(smem_raw);
   const int tid = threadIdx.x, nth = blockDim.x;
   const int i = blockIdx.x;
@@ -216,7 +216,7 @@ __global__ void k_stick_forward(const C* __restrict__ sticks, const C* __restric
                                 const int* __restrict__ stick_map, const R* __restrict__ kin, int n1, int smax,
                                 int npw_pad, int nb, int k0, AxisPlan ap1, const C* __restrict__ tw1, int W,
                                 R scale) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   C* lds = reinterpret_cast<C*>(smem_raw);
   const int tid = threadIdx.x, nth = blockDim.x;
   const int kb = blockIdx.y;
@@ -254,7 +254,7 @@ __global__ void k_density_plane(const C* __restrict__ sticks, const int* __restr
                                 const double* __restrict__ band_weight, int nlist, int chunk, int n1, int n2,
                                 int n3, int smax, int nb, int k0, AxisPlan ap2, AxisPlan ap3,
                                 const C* __restrict__ tw2, const C* __restrict__ tw3, double* __restrict__ part) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   C* plane = reinterpret_cast<C*>(smem_raw);
   const int tid = threadIdx.x, nth = blockDim.x;
   const int i = blockIdx.x;

@@ -107,9 +107,9 @@ __global__ __launch_bounds__(256) void k_plane_vloc_n(C* __restrict__ sticks, co
                                                       int smax, int nb, int k0, const C* __restrict__ tw) {
   constexpr int NP = N * N;
   constexpr int T2 = ceil_div(N * Split<N>::N1, 256);
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   C* plane = reinterpret_cast<C*>(smem_raw);
-  __shared__ int rl[N];
+  __shared__ int rl[(N + 3) / 4 * 4];  // a multiple of 16 bytes: the dynamic plane after it stays 16-byte aligned
   const int tid = threadIdx.x;
   const int i = blockIdx.x;
   const int kb = blockIdx.y;
@@ -145,9 +145,9 @@ __global__ __launch_bounds__(512) void k_density_plane_n(const C* __restrict__ s
   constexpr int NP = N * N;
   constexpr int T2 = ceil_div(N * Split<N>::N1, 512);
   static_assert(MAXPTS * 512 >= NP, "density accumulator too small");
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   C* plane = reinterpret_cast<C*>(smem_raw);
-  __shared__ int rl[N];
+  __shared__ int rl[(N + 3) / 4 * 4];  // a multiple of 16 bytes: the dynamic plane after it stays 16-byte aligned
   const int tid = threadIdx.x;
   const int i = blockIdx.x;
   const int ch = blockIdx.y;

@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from . import ops
-from ._lib import FP64
+from ._lib import FP64, AmtError
 
 _IMPLEMENTED = ("davidson",)
 
@@ -192,7 +192,9 @@ class _Operator:
         if self.host is None:
             try:
                 out = self.fn(block)
-            except (TypeError, RuntimeError, AttributeError):
+            except AmtError:
+                raise
+            except (TypeError, RuntimeError, AttributeError):  # a host-only callable given a device tensor
                 out = None
             if isinstance(out, torch.Tensor):
                 self.host = False

@@ -40,6 +40,20 @@ def _require(t: torch.Tensor, name: str):
         raise AmtError(f"{name} must be contiguous")
 
 
+def _require_rows(t: torch.Tensor, name: str):
+    """A 2-D or 3-D device matrix (batch) whose rows are contiguous and do not overlap."""
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise AmtError(f"{name} must be a HIP device tensor")
+    if t.dim() not in (2, 3):
+        raise AmtError(f"{name} must be 2-D or 3-D")
+    if t.shape[-1] > 1 and t.stride(-1) != 1:
+        raise AmtError(f"{name} must have unit stride along its last axis")
+    if t.shape[-2] > 1 and t.stride(-2) < t.shape[-1]:
+        raise AmtError(f"{name} rows overlap")
+    if t.dim() == 3 and t.shape[0] > 1 and t.stride(0) < t.shape[-2] * t.stride(-2) and t.shape[-2] > 0:
+        raise AmtError(f"{name} batch entries overlap")
+
+
 def workspace(nbytes: int, tag: str = "ws") -> torch.Tensor:
     """A cached device byte buffer of at least ``nbytes`` (grown, never shrunk)."""
     key = (tag, torch.cuda.current_device())
@@ -56,9 +70,11 @@ def zgemm(a: torch.Tensor, b: torch.Tensor, op: int, *, alpha: float = 1.0, beta
 
     ``op = 0``: b is (k, n). ``op = 1``: b is (n, k) and ``conj(b)^T`` is used.
     2-D operands, or 3-D operands batched over the leading axis.
+    Operands may be strided views: each matrix needs unit stride along its rows' elements,
+    a row stride of at least its column count and any batch stride.
     """
-    _require(a, "a")
-    _require(b, "b")
+    _require_rows(a, "a")
+    _require_rows(b, "b")
     if a.dtype != b.dtype or a.dtype not in (torch.complex64, torch.complex128):
         raise AmtError("zgemm operands must share a complex dtype")
     batched = a.dim() == 3
@@ -75,16 +91,17 @@ def zgemm(a: torch.Tensor, b: torch.Tensor, op: int, *, alpha: float = 1.0, beta
         if beta != 0.0:
             raise AmtError("beta != 0 needs out")
         out = torch.empty((nb, m, n) if batched else (m, n), dtype=a.dtype, device=a.device)
-    _require(out, "out")
+    _require_rows(out, "out")
     c3 = out if batched else out.unsqueeze(0)
     if tuple(c3.shape) != (nb, m, n):
         raise AmtError(f"zgemm out has shape {tuple(out.shape)}, expected {(nb, m, n)}")
     prec = precision_of(a)
     wsb = lib().amt_zgemm_workspace(prec, op, m, n, k, nb)
     ws = workspace(wsb, "gemm") if wsb else None
-    check(lib().amt_zgemm(prec, op, m, n, k, float(alpha), a3.data_ptr(), k, m * k, b3.data_ptr(),
-                          b3.shape[2], b3.shape[1] * b3.shape[2], float(beta), c3.data_ptr(), n, m * n, nb,
-                          ws.data_ptr() if ws is not None else None, int(wsb), stream_handle()), "zgemm")
+    check(lib().amt_zgemm(prec, op, m, n, k, float(alpha), a3.data_ptr(), a3.stride(1), a3.stride(0),
+                          b3.data_ptr(), b3.stride(1), b3.stride(0), float(beta), c3.data_ptr(), c3.stride(1),
+                          c3.stride(0), nb, ws.data_ptr() if ws is not None else None, int(wsb), stream_handle()),
+          "zgemm")
     return out
 
 

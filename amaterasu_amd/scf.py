@@ -26,6 +26,7 @@ import numpy as np
 import torch
 
 from . import ops
+from .batched import solve_batched
 from .density import _trivial_symmetry
 from .eigen import davidson, orthonormalize
 from .hamiltonian import Hamiltonian, device_plan
@@ -56,6 +57,9 @@ class SCFSettings:
     #: FP32 blocks cannot resolve residual norms below a few 1e-7 ||H||; the eigensolver
     #: tolerance is floored here while they are in use.
     fp32_tol_floor: float = 2e-6
+    #: Run the Davidson of all k-points of a spin channel together (one launch per kernel
+    #: over every k); each k-point still follows its own iteration.
+    batched_k: bool = True
 
 
 @dataclass
@@ -353,6 +357,14 @@ def run_scf(system, settings: SCFSettings, *, density=None, wavefunctions=None, 
             tol = max(tol, settings.fp32_tol_floor)
         steps = 40 if it == 1 else 20
         for sp in range(nspin):
+            if settings.batched_k and nk > 1:
+                kin_b = plan.kinetic[1 if cdt == torch.complex128 else 0]
+                r = solve_batched(lambda b, sp=sp: ham.apply_batched(b, sp), wf[sp].contiguous(), kin_b,
+                                  plan.npw, tolerance=tol, max_iterations=steps)
+                eig[sp] = r.eigenvalues
+                wf[sp] = r.coefficients.to(cdt)
+                dav_steps += int(r.iterations.sum())
+                continue
             for ik in range(nk):
                 n = plan.npw[ik]
                 r = davidson(lambda b, ik=ik, sp=sp: ham.apply(b, ik, sp), wf[sp, ik, :, :n].contiguous(),

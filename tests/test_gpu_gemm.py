@@ -58,3 +58,24 @@ def test_mfma_lane_layout_single_tile():
     for dt in (torch.complex128, torch.complex64):
         got = ops.zgemm(torch.from_numpy(a).to("cuda", dt), torch.from_numpy(b).to("cuda", dt), 0).cpu().numpy()
         assert np.array_equal(got.astype(np.complex128), a @ b)
+
+
+def test_zgemm_strided_views():
+    """Row-strided and batch-strided views (a prefix of a larger subspace block) go to the
+    kernel without copies and give the same products."""
+    import torch
+
+    from amaterasu_amd import ops
+
+    g = torch.Generator().manual_seed(5)
+    big = torch.randn((3, 40, 130), dtype=torch.complex128, generator=g).cuda()
+    a = torch.randn((3, 7, 130), dtype=torch.complex128, generator=g).cuda()
+    view = big[:, :25]  # batch stride 40*130, rows of 130
+    c = ops.zgemm(a, view, 1)
+    want = a.cpu() @ view.cpu().conj().transpose(-1, -2)
+    assert torch.allclose(c.cpu(), want, rtol=1e-12, atol=1e-11)
+    rot = torch.randn((3, 5, 25), dtype=torch.complex128, generator=g).cuda()
+    out = torch.zeros((3, 9, 130), dtype=torch.complex128).cuda()
+    ops.zgemm(rot, view, 0, out=out[:, 2:7])
+    assert torch.allclose(out[:, 2:7].cpu(), rot.cpu() @ view.cpu(), rtol=1e-12, atol=1e-11)
+    assert float(out[:, :2].abs().max()) == 0.0 and float(out[:, 7:].abs().max()) == 0.0
