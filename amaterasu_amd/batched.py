@@ -74,7 +74,7 @@ def _cholesky_rows(block: torch.Tensor, used: torch.Tensor):
     """L^{-1} block per k (rows >= used[k] are zero and stay zero); None where it fails."""
     s = _pad_diag(_herm(_bgram(block, block)), used, 1.0)
     low, info = torch.linalg.cholesky_ex(s)
-    eye = torch.eye(s.shape[-1], dtype=s.dtype, device=s.device).expand_as(s)
+    eye = torch.eye(s.shape[-1], dtype=s.dtype, device=s.device).repeat(s.shape[0], 1, 1)
     linv = torch.linalg.solve_triangular(low, eye, upper=False)
     return _brot(linv, block), info
 

@@ -2,7 +2,10 @@
 device Davidson, which the Davidson tests pin to the oracle and to dense eigh.
 
 Each k-point follows its own iteration in both, so converged eigenpairs agree to the
-eigensolver tolerance; ragged plane-wave counts (+-8 %) exercise the padding."""
+eigensolver tolerance; ragged plane-wave counts (+-8 %) exercise the padding. The
+reference iteration (collapse to the Ritz vectors, no thick restart) converges the top
+band of a block slowly, so the cases use the 16 valence bands of Si8 (a gap above the
+block) and tolerances it reaches in well under the iteration limit."""
 
 from __future__ import annotations
 
@@ -18,14 +21,14 @@ def _ragged_system(precision="fp64"):
     from amaterasu_amd.system import build_system, load_dataset
     from amaterasu_amd.workloads import si8_kbatch, smooth_potential
 
-    spec = si8_kbatch(nk=4, nbands=12, ecut=8.0, spread=0.08)
+    spec = si8_kbatch(nk=4, nbands=16, ecut=8.0, spread=0.08)
     s = build_system(spec.cell, spec.positions, spec.symbols, {"Si": load_dataset()}, spec.ecut_rho, spec.kpoints,
                      spec.ecuts)
-    v = smooth_potential(s.grid.shape, seed=spec.seed, rms=0.3)
+    v = smooth_potential(s.grid.shape, seed=spec.seed, rms=0.05)
     return s, Hamiltonian(s, v, precision=precision)
 
 
-@pytest.mark.parametrize("precision,tol,eig_tol", [("fp64", 1e-9, 1e-9), ("fp32", 2e-5, 5e-6)])
+@pytest.mark.parametrize("precision,tol,eig_tol", [("fp64", 1e-6, 1e-10), ("fp32", 2e-5, 5e-6)])
 def test_batched_matches_per_k(precision, tol, eig_tol):
     from amaterasu_amd.batched import solve_batched
     from amaterasu_amd.eigen import davidson, random_bands
@@ -34,21 +37,21 @@ def test_batched_matches_per_k(precision, tol, eig_tol):
     plan = ham.plan
     assert len(set(plan.npw)) == len(plan.npw)  # ragged
     cdt = torch.complex128 if precision == "fp64" else torch.complex64
-    nb = 12
+    nb = 16
     guess = torch.zeros((plan.nk, nb, plan.npw_pad), dtype=cdt, device="cuda")
     for k, sph in enumerate(s.wave_spheres):
         guess[k, :, : len(sph.g2)] = random_bands(len(sph.g2), nb, sph.g2 / 2, seed=7 + k, device=True).to(cdt)
     kin = plan.kinetic[1 if precision == "fp64" else 0]
-    res = solve_batched(lambda b: ham.apply_batched(b), guess, kin, plan.npw, tolerance=tol, max_iterations=60)
+    res = solve_batched(lambda b: ham.apply_batched(b), guess, kin, plan.npw, tolerance=tol, max_iterations=100)
     assert res.converged.all()
     for k, sph in enumerate(s.wave_spheres):
         n = len(sph.g2)
         ref = davidson(lambda b, k=k: ham.apply(b, k), guess[k, :, :n].contiguous(), kin[k, :n], tolerance=tol,
-                       max_iterations=60, precision=precision)
+                       max_iterations=100, precision=precision)
         assert ref.converged
         assert torch.abs(res.eigenvalues[k] - ref.eigenvalues).max().item() < eig_tol
         c = res.coefficients[k].to(torch.complex128)
-        assert float(c[:, n:].abs().max()) == 0.0  # padding stays zero
+        assert c[:, n:].numel() == 0 or float(c[:, n:].abs().max()) == 0.0  # padding stays zero
         ov = c[:, :n] @ c[:, :n].conj().T
         assert torch.abs(ov - torch.eye(nb, dtype=ov.dtype, device=ov.device)).max().item() < 1e-5 * (
             1 if precision == "fp32" else 1e-6)
@@ -63,16 +66,17 @@ def test_batched_frozen_kpoint_keeps_its_answer():
 
     s, ham = _ragged_system()
     plan = ham.plan
-    nb = 8
+    nb = 16
     guess = torch.zeros((plan.nk, nb, plan.npw_pad), dtype=torch.complex128, device="cuda")
     for k, sph in enumerate(s.wave_spheres):
         guess[k, :, : len(sph.g2)] = random_bands(len(sph.g2), nb, sph.g2 / 2, seed=3 + k, device=True)
     kin = plan.kinetic[1]
     n0 = plan.npw[0]
-    pre = davidson(lambda b: ham.apply(b, 0), guess[0, :, :n0].contiguous(), kin[0, :n0], tolerance=1e-11,
-                   max_iterations=80)
+    pre = davidson(lambda b: ham.apply(b, 0), guess[0, :, :n0].contiguous(), kin[0, :n0], tolerance=1e-7,
+                   max_iterations=300)
+    assert pre.converged
     guess[0, :, :n0] = pre.coefficients
-    res = solve_batched(lambda b: ham.apply_batched(b), guess, kin, plan.npw, tolerance=1e-9, max_iterations=60)
+    res = solve_batched(lambda b: ham.apply_batched(b), guess, kin, plan.npw, tolerance=1e-6, max_iterations=100)
     assert res.iterations[0] == 1
     assert res.iterations[1:].min() > 1
-    assert torch.abs(res.eigenvalues[0] - pre.eigenvalues).max().item() < 1e-10
+    assert torch.abs(res.eigenvalues[0] - pre.eigenvalues).max().item() < 1e-12
