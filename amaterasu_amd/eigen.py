@@ -208,17 +208,31 @@ class _Operator:
 
 def davidson(apply_hamiltonian, guess, kinetic, *, tolerance: float = 1e-8, max_iterations: int = 40,
              subspace_factor: int = 4, precision="fp64") -> DavidsonResult:
-    """Lowest eigenpairs of a Hermitian operator from a starting block (``davidson.py:48-111``)."""
+    """Lowest eigenpairs of a Hermitian operator from a starting block (``davidson.py:48-111``).
+
+    The basis and H.basis live in preallocated ``[max_subspace][npw]`` device buffers, and
+    the projected matrix of the Rayleigh-Ritz step is kept between passes: each pass adds
+    only the columns of the new directions (8 m n_new npw flops instead of 8 m^2 npw) --
+    the same matrix the reference rebuilds from scratch (``subspace.py:72``), up to
+    rounding. After a collapse the block of the Ritz vectors is recomputed.
+    """
     host = not isinstance(guess, torch.Tensor)
     cdt = torch.complex128 if precision in ("fp64", FP64, None) else torch.complex64
     op = _Operator(apply_hamiltonian, cdt)
     kin = torch.as_tensor(np.asarray(kinetic, dtype=float) if not isinstance(kinetic, torch.Tensor) else kinetic,
                           dtype=torch.float64).to("cuda")
-    basis = orthonormalize(_to_device(guess, cdt))
-    nbands, npw = basis.shape
+    v0 = orthonormalize(_to_device(guess, cdt))
+    nbands, npw = v0.shape
     max_subspace = min(max(subspace_factor, 2) * nbands, npw)
-    h_basis = op(basis)
-    eps, vec, hvec = rayleigh_ritz(basis, h_basis, nbands)
+    cap = max(max_subspace, 2 * nbands)
+    basis = torch.zeros((cap, npw), dtype=cdt, device="cuda")
+    h_basis = torch.zeros_like(basis)
+    proj = torch.zeros((cap, cap), dtype=torch.complex128, device="cuda")
+    basis[:nbands] = v0
+    h_basis[:nbands] = op(v0)
+    m = nbands
+    proj[:m, :m] = _hermitian(_gram(basis[:m], h_basis[:m]).conj())
+    eps, vec, hvec = _ritz(proj[:m, :m], basis[:m], h_basis[:m], nbands)
     norms = torch.full((nbands,), float("inf"), dtype=torch.float64, device="cuda")
     iteration = 0
     for step in range(1, max_iterations + 1):
@@ -228,19 +242,36 @@ def davidson(apply_hamiltonian, guess, kinetic, *, tolerance: float = 1e-8, max_
         if active.numel() == 0:
             break
         corr = ops.precondition(res, vec, kin, active)
-        new = orthogonalize_against(corr, basis)
-        if new.shape[0] == 0:
+        new = orthogonalize_against(corr, basis[:m])
+        nn = new.shape[0]
+        if nn == 0:
             break
-        basis = torch.cat([basis, new])
-        h_basis = torch.cat([h_basis, op(new)])
-        eps, vec, hvec = rayleigh_ritz(basis, h_basis, nbands)
-        if basis.shape[0] + nbands > max_subspace:
-            basis, h_basis = vec, hvec
+        nn = min(nn, cap - m)
+        basis[m : m + nn] = new[:nn]
+        h_basis[m : m + nn] = op(new[:nn].contiguous())
+        cols = _gram(basis[: m + nn], h_basis[m : m + nn]).conj()  # proj[:, new]
+        proj[:m, m : m + nn] = cols[:m]
+        proj[m : m + nn, :m] = cols[:m].conj().T
+        proj[m : m + nn, m : m + nn] = _hermitian(cols[m:])
+        m += nn
+        eps, vec, hvec = _ritz(proj[:m, :m], basis[:m], h_basis[:m], nbands)
+        if m + nbands > max_subspace:
+            basis[:nbands] = vec
+            h_basis[:nbands] = hvec
+            m = nbands
+            proj[:m, :m] = _hermitian(_gram(basis[:m], h_basis[:m]).conj())
     converged = bool((norms <= tolerance).all().item())
     if host:
         return DavidsonResult(eps.cpu().numpy(), vec.to(torch.complex128).cpu().numpy(), norms.cpu().numpy(),
                               iteration, converged)
     return DavidsonResult(eps, vec, norms, iteration, converged)
+
+
+def _ritz(proj: torch.Tensor, basis: torch.Tensor, h_basis: torch.Tensor, nbands: int):
+    """Lowest ``nbands`` eigenpairs of the projected matrix, rotated into the basis."""
+    w, vec = torch.linalg.eigh(proj)
+    rot = vec[:, :nbands].T.contiguous()
+    return w[:nbands], _rotate(rot, basis), _rotate(rot, h_basis)
 
 
 def solve(name, apply_hamiltonian, guess, kinetic, *, apply_overlap=None, **kwargs) -> DavidsonResult:
