@@ -56,6 +56,7 @@ class DeviceSystem:
     v_local: object = None
     core_density: object = None
     ewald: float = 0.0
+    datasets: dict = field(default_factory=dict)
 
     def initial_density(self):
         """Superposed atomic valence densities (``hamiltonian/system.py:147-161``), on the device."""
@@ -213,6 +214,7 @@ def attach_ionic_fields(system: DeviceSystem, datasets: dict, ecut_rho: float, x
     import torch
 
     syms = system.symbols
+    system.datasets = dict(datasets)
     system.ecut_rho = float(ecut_rho)
     system.xc = xc
     system.v_local = superpose(system.grid, ecut_rho, system.positions, syms,

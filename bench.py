@@ -10,9 +10,12 @@ band-applications per second in the mixed-precision (FP32) form; the FP64 form i
 reported alongside. The ``roofline`` object covers the dominant kernel (the per-plane
 2D FFT / V_eff kernel), timed with HIP events on the stream it runs on; the
 ``cpu_baseline`` object times the FP64 numpy oracle of the same operator on a bounded
-sample of the same workload on this host.
+sample of the same workload on this host. The ``si64_scf`` object is the other half of
+the metric: wall time of the Si64 Gamma-point SCF to 1e-6 Ha/atom in plain FP64 and in
+mixed precision from the learned start, with the energy and eigenvalue differences.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-scf] [--no-cpu-baseline]
+    python bench.py --pmc-probe      # short program for rocprofv3 --pmc runs
 """
 
 from __future__ import annotations
@@ -133,6 +136,76 @@ def time_kernels(ham, psi, reps: int = 20):
     return res
 
 
+def si64_scf(log=None) -> dict:
+    """Wall time of the Si64 Gamma-point SCF to 1e-6 Ha/atom (and density residual 1e-6):
+    plain FP64 from the atomic superposition against mixed precision from the learned start
+    (the atomic start when no trained model is present). The learned start's own cost
+    (features, network, clip and rescale) is inside its timed region; the system setup,
+    common to both, is not."""
+    import numpy as np
+    import torch
+
+    from amaterasu_amd.learned import MODEL_PATH, DensityModel, learned_density
+    from amaterasu_amd.scf import HARTREE_IN_EV, SCFSettings, run_scf
+    from amaterasu_amd.workloads import si64_scf_system
+
+    t0 = time.perf_counter()
+    system = si64_scf_system()
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    tol = dict(tol_energy=1e-6 * HARTREE_IN_EV, tol_density=1e-6, max_steps=60)
+    out = {}
+    learned = MODEL_PATH.exists()
+    model = DensityModel.load() if learned else None
+    for name, prec in (("fp64", "fp64"), ("mixed_learned" if learned else "mixed", "mixed")):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rho = learned_density(system, model) if (prec == "mixed" and learned) else system.initial_density()
+        r = run_scf(system, SCFSettings(precision=prec, **tol), density=rho)
+        torch.cuda.synchronize()
+        out[name] = {"seconds": round(time.perf_counter() - t, 3), "iterations": r.iterations,
+                     "davidson_steps": r.davidson_steps, "converged": r.converged, "energy_ha": r.energies.total,
+                     "_eig": r.eigenvalues}
+        if log:
+            log(f"si64 scf {name}: {out[name]['seconds']} s, {r.iterations} iterations")
+    a, b = out["fp64"], out["mixed_learned" if learned else "mixed"]
+    res = {"workload": "Si64 2x2x2 conventional (a=5.431 A, seeded displacements sigma 0.05 A), PBE, ecut 20 Ha, "
+                       "84^3, Gamma, 160 bands, tol 1e-6 Ha/atom + density residual 1e-6",
+           "setup_seconds": round(setup, 3), "learned_start": learned,
+           "speedup": round(a["seconds"] / b["seconds"], 3),
+           "energy_difference_ha_per_atom": abs(a["energy_ha"] - b["energy_ha"]) / 64,
+           "eigenvalue_max_difference_ha": float(np.abs(a["_eig"] - b["_eig"]).max())}
+    for k in list(out):
+        out[k].pop("_eig")
+        res[k] = out[k]
+    return res
+
+
+def pmc_probe() -> None:
+    """Three launches of the plane kernel on the k-batch workload and one 256 MiB device
+    copy, for the FETCH_SIZE / WRITE_SIZE counter runs (the copy scales the counters)."""
+    import torch
+
+    from amaterasu_amd import ops
+    from amaterasu_amd._lib import check, lib
+
+    _, _, _, ham, psi = build_workload("fp32")
+    plan = ham.plan
+    nkk, nb, _ = psi.shape
+    out = torch.zeros_like(psi)
+    st = plan.sticks(0, nkk * nb)
+    v = ham.veff(0, 0)
+    for _ in range(3):
+        for i in (1, 2):
+            check(lib().amt_hpsi_stage(plan.handle, 0, i, psi.data_ptr(), out.data_ptr(), nb, 0, nkk, v.data_ptr(),
+                                       st.data_ptr(), ops.stream_handle()))
+    src = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print("pmc probe done: 3 x (stick_inverse, plane_vloc) + one 268435456-byte copy", flush=True)
+
+
 def cpu_baseline(spec, v, seconds: float = 15.0):
     """FP64 numpy oracle of Hamiltonian.apply on a bounded sample of the same workload."""
     import numpy as np
@@ -178,6 +251,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-scf", action="store_true", help="skip the Si64 SCF wall-time comparison")
+    ap.add_argument("--pmc-probe", action="store_true", help="only the short program for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the k-point/band sharding across "
@@ -190,6 +265,9 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs an MI355X (HIP device)", file=sys.stderr)
         return 1
+    if args.pmc_probe:
+        pmc_probe()
+        return 0
 
     spec, system, v, ham32, psi32 = build_workload("fp32")
     nkb = psi32.shape[0] * psi32.shape[1]
@@ -246,6 +324,8 @@ def main() -> int:
         "fp32_vs_fp64_max_rel_error": rel32,
         "kernels_ms_fp32": {k: round(x, 5) for k, x in kern32.items()},
     }
+    if not args.no_scf:
+        line["si64_scf"] = si64_scf(log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec, v, args.cpu_seconds)
     print(json.dumps(line), flush=True)
