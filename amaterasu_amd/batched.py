@@ -11,9 +11,9 @@ Ragged shapes are padded, never masked by branching on the device:
 * plane waves: every row is ``npw_pad`` long with zeros past the k-point's own count
   (H.psi, the GEMMs and the preconditioner keep the padding zero);
 * subspace rows: k-point k uses the first m_k rows of a ``[nk][M][npw_pad]`` basis; the
-  unused rows are zero, and their diagonal of every projected (Gram) matrix is set to a
-  large value (to one), so the padded eigenpairs sort above every physical one (the
-  Cholesky of the padded block is the identity) and never mix with them;
+  unused rows are zero, and their diagonal of every projected matrix is set to distinct
+  large values (of a Gram matrix, to distinct negative values; before a Cholesky, to one),
+  so the padded eigenpairs sort apart from every physical one and never mix with them;
 * new directions: k-point k contributes n_k <= nb rows of a zero-padded block.
 
 A k-point that has converged (or that stopped adding directions) is frozen: its
@@ -60,13 +60,15 @@ def _brot(rot: torch.Tensor, block: torch.Tensor) -> torch.Tensor:
     return ops.zgemm(rot.to(block.dtype).contiguous(), block, 0)
 
 
-def _pad_diag(m: torch.Tensor, used: torch.Tensor, value: float) -> torch.Tensor:
-    """Set the diagonal of rows/cols >= used[k] to ``value`` (those rows/cols are zero)."""
+def _pad_diag(m: torch.Tensor, used: torch.Tensor, value: float, step: float = 0.0) -> torch.Tensor:
+    """Set the diagonal of rows/cols j >= used[k] (zero rows/cols) to ``value + step * j``:
+    distinct values keep the padded block free of repeated eigenvalues, which the batched
+    Hermitian eigensolver does not always resolve."""
     n = m.shape[-1]
     idx = torch.arange(n, device=m.device)
     mask = idx[None, :] >= used[:, None]  # [nk][n]
     diag = torch.diagonal(m, dim1=-2, dim2=-1)
-    diag += mask.to(m.dtype) * value
+    diag += mask.to(m.dtype) * (value + step * idx.to(m.dtype))[None, :]
     return m
 
 
@@ -79,11 +81,42 @@ def _cholesky_rows(block: torch.Tensor, used: torch.Tensor):
     return _brot(linv, block), info
 
 
+def _eigh(m: torch.Tensor):
+    """Hermitian eigen-decomposition of each matrix of a batch, one solver call per matrix
+    (the batched solver's Jacobi sweeps do not always converge on ill-conditioned Gram
+    matrices; the single-matrix divide-and-conquer solver does)."""
+    if m.shape[0] == 1:
+        w, u = torch.linalg.eigh(m[0])
+        return w[None], u[None]
+    pairs = [torch.linalg.eigh(x) for x in m]
+    return torch.stack([p[0] for p in pairs]), torch.stack([p[1] for p in pairs])
+
+
 def _gram_rows(block: torch.Tensor, used: torch.Tensor, threshold: float):
     """Per k: keep directions with singular value > threshold * s_max, orthonormal
-    (the Gram-matrix route of ``eigen.orthogonalize_against``). Returns (rows, kept)."""
+    (the routes of ``eigen.orthogonalize_against``). Returns (rows, kept)."""
     g = _herm(_bgram(block, block))
-    w, u = torch.linalg.eigh(g)  # ascending; padded zero rows give zero eigenvalues
+    gp = _pad_diag(g.clone(), used, 1.0)
+    low, info = torch.linalg.cholesky_ex(gp)
+    eye = torch.eye(g.shape[-1], dtype=g.dtype, device=g.device).repeat(g.shape[0], 1, 1)
+    linv = torch.linalg.solve_triangular(low, eye, upper=False)
+    # the padded identity block adds at most sqrt(pad) to both Frobenius norms; use the
+    # used block alone: zero the padded rows/cols before measuring
+    n = g.shape[-1]
+    live = (torch.arange(n, device=g.device)[None, :] < used[:, None]).to(g.dtype)
+    mask = live[:, :, None] * live[:, None, :]
+    lo = 1.0 / torch.linalg.matrix_norm(linv * mask)
+    hi = torch.linalg.matrix_norm(low * mask)
+    ok = (used == 0) | ((info == 0) & torch.isfinite(lo) & (lo > threshold * hi))
+    if bool(ok.all().item()):
+        v = _brot(linv, block)
+        again, info2 = _cholesky_rows(v, used)
+        good = info2 == 0
+        return torch.where(good[:, None, None], again, v), used
+    # padded rows get distinct negative eigenvalues: they sort last after the flip, and
+    # s = sqrt(max(w, 0)) = 0 drops them
+    g = _pad_diag(g, used, -1.0, -1.0)
+    w, u = _eigh(g)
     w = torch.flip(w, dims=[-1])
     u = torch.flip(u, dims=[-1])
     s = torch.sqrt(torch.clamp(w, min=0.0))
@@ -102,8 +135,8 @@ def _gram_rows(block: torch.Tensor, used: torch.Tensor, threshold: float):
 
 def _rayleigh_ritz(basis, h_basis, used, nbands):
     m = _bgram(basis, h_basis)  # conj of the projected matrix
-    proj = _pad_diag(_herm(m.conj()), used, _PAD_EIG)
-    w, vec = torch.linalg.eigh(proj)
+    proj = _pad_diag(_herm(m.conj()), used, _PAD_EIG, 1.0)
+    w, vec = _eigh(proj)
     rot = vec[:, :, :nbands].transpose(-1, -2).contiguous()
     return w[:, :nbands].contiguous(), _brot(rot, basis), _brot(rot, h_basis)
 

@@ -8,11 +8,13 @@ matrix and every rotation is an MFMA complex GEMM. The small dense factorisation
 (Cholesky, Hermitian eigensolve) stay in FP64 on the GPU (torch.linalg on ROCm).
 
 One step differs in method, not in result: ``orthogonalize_against`` orthonormalises
-the projected block through the eigen-decomposition of its Gram matrix followed by a
-Cholesky re-orthonormalisation (CholeskyQR2), where the reference takes a thin SVD of
-the (rows x npw) block (``subspace.py:56-60``). Both keep exactly the directions whose
-singular value exceeds ``threshold * s_max`` and return an orthonormal basis of their
-span; the Gram route avoids a tall-skinny SVD on the device.
+the projected block by CholeskyQR2 when the Cholesky factor of its Gram matrix proves
+that every singular value exceeds ``threshold * s_max``, and otherwise through the
+eigen-decomposition of the Gram matrix followed by a Cholesky re-orthonormalisation,
+where the reference takes a thin SVD of the (rows x npw) block (``subspace.py:56-60``).
+All routes keep exactly the directions whose singular value exceeds
+``threshold * s_max`` and return an orthonormal basis of their span (the Rayleigh-Ritz
+step that follows depends on the span only); none needs a tall-skinny SVD on the device.
 
 ``apply_hamiltonian`` may be the reference's closure over a :class:`Hamiltonian` of this
 package (device tensors flow through) or any numpy callable (blocks go through the host).
@@ -88,14 +90,36 @@ def _cholesky_orthonormalize(block: torch.Tensor):
     return out
 
 
+def _cholesky_keeps_all(g: torch.Tensor, threshold: float):
+    """L^{-1} when the Cholesky factor proves that every singular value of the block
+    exceeds ``threshold`` times the largest (then every direction is kept), else None.
+
+    With G = B B^H = L L^H the singular values of B are those of L, and
+    s_min(L) >= 1 / ||L^{-1}||_F, s_max(L) <= ||L||_F: the test is rigorous, not a guess.
+    """
+    lower, info = torch.linalg.cholesky_ex(g)
+    eye = torch.eye(g.shape[-1], dtype=g.dtype, device=g.device)
+    linv = torch.linalg.solve_triangular(lower, eye, upper=False)
+    lo = 1.0 / torch.linalg.matrix_norm(linv)
+    ok = (info == 0) & torch.isfinite(lo) & (lo > threshold * torch.linalg.matrix_norm(lower))
+    return linv if bool(ok.item()) else None
+
+
 def _gram_orthonormalize(block: torch.Tensor, threshold: float) -> torch.Tensor:
     """Rows spanning the directions with singular value > threshold * s_max, orthonormal.
 
-    s^2 and the right singular vectors come from the eigen-decomposition of the Gram
-    matrix (descending, as an SVD orders them); one CholeskyQR pass restores full
-    orthonormality for the weakest kept directions.
+    When the Cholesky factor of the Gram matrix proves that no singular value falls
+    below the threshold, the block is orthonormalised by two Cholesky passes
+    (CholeskyQR2) and spans the same subspace the SVD route keeps. Otherwise s^2 and the
+    right singular vectors come from the eigen-decomposition of the Gram matrix
+    (descending, as an SVD orders them), followed by one CholeskyQR pass.
     """
     g = _hermitian(_gram(block, block))
+    linv = _cholesky_keeps_all(g, threshold)
+    if linv is not None:
+        v = _rotate(linv, block)
+        again = _cholesky_orthonormalize(v)
+        return v if again is None else again
     w, u = torch.linalg.eigh(g)
     w = torch.flip(w, dims=[0])
     u = torch.flip(u, dims=[1])

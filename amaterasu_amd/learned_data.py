@@ -19,6 +19,7 @@ exchange-correlation with the repository's norm-conserving Si dataset.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import time
 from pathlib import Path
@@ -55,14 +56,15 @@ def training_system(index: int, ds=None):
     return attach_ionic_fields(s, {"Si": ds}, 4 * ECUT, xc="LDA-PW")
 
 
-def generate(first: int, count: int, out: Path, log=print) -> None:
+def generate(first: int, count: int, out: Path, log=print, *, batched_k: bool = True) -> None:
     out.mkdir(parents=True, exist_ok=True)
     ds = load_dataset()
+    settings = dataclasses.replace(TOL, batched_k=batched_k)
     for c in range(first, first + count):
         t0 = time.perf_counter()
         s = training_system(c, ds)
         atomic = s.initial_density()
-        r = run_scf(s, TOL)
+        r = run_scf(s, settings)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         corr = (torch.from_numpy(r.density).to("cuda") - atomic).sum(0)
@@ -154,6 +156,7 @@ def main(argv=None) -> int:
     g.add_argument("--first", type=int, default=0)
     g.add_argument("--count", type=int, default=320)
     g.add_argument("--out", type=Path, required=True)
+    g.add_argument("--per-k", action="store_true", help="k-by-k Davidson instead of the k-batched one")
     t = sub.add_parser("train")
     t.add_argument("--data", type=Path, nargs="+", required=True)
     t.add_argument("--out", type=Path, required=True)
@@ -167,7 +170,7 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)  # noqa: E731
     if a.cmd == "generate":
-        generate(a.first, a.count, a.out, log)
+        generate(a.first, a.count, a.out, log, batched_k=not a.per_k)
     elif a.cmd == "train":
         train_model(a.data, a.out, ntrain=a.ntrain, nheld=a.nheld, epochs=a.epochs, log=log)
     else:
