@@ -11,9 +11,9 @@ Ragged shapes are padded, never masked by branching on the device:
 * plane waves: every row is ``npw_pad`` long with zeros past the k-point's own count
   (H.psi, the GEMMs and the preconditioner keep the padding zero);
 * subspace rows: k-point k uses the first m_k rows of a ``[nk][M][npw_pad]`` basis; the
-  unused rows are zero, and their diagonal of every projected matrix is set to distinct
-  large values (of a Gram matrix, to distinct negative values; before a Cholesky, to one),
-  so the padded eigenpairs sort apart from every physical one and never mix with them;
+  unused rows are zero. Each small eigensolve runs on the k-point's own m_k x m_k block
+  and is padded afterwards (unit eigenvectors on the unused coordinates, eigenvalues that
+  sort them out of the way); before a Cholesky the padded diagonal is set to one;
 * new directions: k-point k contributes n_k <= nb rows of a zero-padded block.
 
 A k-point that has converged (or that stopped adding directions) is frozen: its
@@ -81,15 +81,29 @@ def _cholesky_rows(block: torch.Tensor, used: torch.Tensor):
     return _brot(linv, block), info
 
 
-def _eigh(m: torch.Tensor):
-    """Hermitian eigen-decomposition of each matrix of a batch, one solver call per matrix
-    (the batched solver's Jacobi sweeps do not always converge on ill-conditioned Gram
-    matrices; the single-matrix divide-and-conquer solver does)."""
-    if m.shape[0] == 1:
-        w, u = torch.linalg.eigh(m[0])
-        return w[None], u[None]
-    pairs = [torch.linalg.eigh(x) for x in m]
-    return torch.stack([p[0] for p in pairs]), torch.stack([p[1] for p in pairs])
+def _eigh(m: torch.Tensor, used, fill: float):
+    """Eigen-decomposition of the leading used[k] x used[k] block of each matrix, one solver
+    call per matrix at its own size, returned padded to the batch size: eigenvalue ``fill``
+    and unit eigenvectors on the padded coordinates, so padded pairs sort after (fill
+    large) or before the physical ones and never mix with them. (The batched solver's
+    Jacobi sweeps do not always converge on ill-conditioned Gram matrices, and the
+    divide-and-conquer solver has returned non-finite eigenvectors for blocks padded with
+    large diagonal values.)"""
+    nk, n = m.shape[0], m.shape[-1]
+    used = [int(u) for u in (used.tolist() if isinstance(used, torch.Tensor) else used)]
+    w = torch.full((nk, n), float(fill), dtype=torch.float64, device=m.device)
+    u = torch.zeros_like(m)
+    for k in range(nk):
+        c = used[k]
+        if c < n:
+            idx = torch.arange(c, n, device=m.device)
+            u[k, idx, idx] = 1.0
+        if c > 0:
+            wk, uk = torch.linalg.eigh(m[k, :c, :c])
+            w[k, :c] = wk
+            u[k, :c, :c] = uk
+    order = torch.argsort(w, dim=-1, stable=True)
+    return torch.gather(w, 1, order), torch.gather(u, 2, order[:, None, :].expand(-1, n, -1))
 
 
 def _gram_rows(block: torch.Tensor, used: torch.Tensor, threshold: float):
@@ -113,10 +127,9 @@ def _gram_rows(block: torch.Tensor, used: torch.Tensor, threshold: float):
         again, info2 = _cholesky_rows(v, used)
         good = info2 == 0
         return torch.where(good[:, None, None], again, v), used
-    # padded rows get distinct negative eigenvalues: they sort last after the flip, and
+    # padded rows get negative eigenvalues: they sort last after the flip, and
     # s = sqrt(max(w, 0)) = 0 drops them
-    g = _pad_diag(g, used, -1.0, -1.0)
-    w, u = _eigh(g)
+    w, u = _eigh(g, used, -1.0)
     w = torch.flip(w, dims=[-1])
     u = torch.flip(u, dims=[-1])
     s = torch.sqrt(torch.clamp(w, min=0.0))
@@ -135,8 +148,8 @@ def _gram_rows(block: torch.Tensor, used: torch.Tensor, threshold: float):
 
 def _rayleigh_ritz(basis, h_basis, used, nbands):
     m = _bgram(basis, h_basis)  # conj of the projected matrix
-    proj = _pad_diag(_herm(m.conj()), used, _PAD_EIG, 1.0)
-    w, vec = _eigh(proj)
+    proj = _herm(m.conj())
+    w, vec = _eigh(proj, used, _PAD_EIG)
     rot = vec[:, :, :nbands].transpose(-1, -2).contiguous()
     return w[:, :nbands].contiguous(), _brot(rot, basis), _brot(rot, h_basis)
 

@@ -182,8 +182,9 @@ def si64_scf(log=None) -> dict:
 
 
 def pmc_probe() -> None:
-    """Three launches of the plane kernel on the k-batch workload and one 256 MiB device
-    copy, for the FETCH_SIZE / WRITE_SIZE counter runs (the copy scales the counters)."""
+    """Three launches of the plane kernel on the k-batch workload and one elementwise kernel
+    that reads and writes 256 MiB, for the FETCH_SIZE / WRITE_SIZE counter runs (the
+    elementwise kernel calibrates the counters)."""
     import torch
 
     from amaterasu_amd import ops
@@ -199,11 +200,11 @@ def pmc_probe() -> None:
         for i in (1, 2):
             check(lib().amt_hpsi_stage(plan.handle, 0, i, psi.data_ptr(), out.data_ptr(), nb, 0, nkk, v.data_ptr(),
                                        st.data_ptr(), ops.stream_handle()))
-    src = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
-    dst.copy_(src)
+    torch.mul(src, 2.0, out=dst)  # the only float32 scale kernel of the program: reads and writes 256 MiB
     torch.cuda.synchronize()
-    print("pmc probe done: 3 x (stick_inverse, plane_vloc) + one 268435456-byte copy", flush=True)
+    print("pmc probe done: 3 x (stick_inverse, plane_vloc) + one 256 MiB -> 256 MiB float32 scale", flush=True)
 
 
 def cpu_baseline(spec, v, seconds: float = 15.0):
