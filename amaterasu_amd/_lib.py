@@ -42,6 +42,8 @@ SIGNATURES = {
     "amt_residual": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp]),
     "amt_precondition": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp]),
     "amt_kb_dij": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_ll, _c_ll, _c_ll, _c_int, _vp]),
+    "amt_heev_workspace": (_c_sz, [_c_int]),
+    "amt_heev_lowest": (_c_int, [_c_int, _vp, _c_int, _vp, _vp, _vp, _c_sz, _vp]),
     "amt_mlp_param_count": (_c_int, []),
     "amt_mlp_features": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _ip, _c_dbl, _c_dbl, _vp, _vp]),
     "amt_mlp_forward": (_c_int, [_vp, _c_ll, _vp, _vp, _vp, _vp, _vp]),

@@ -18,7 +18,8 @@ OUT = HERE / "libamaterasu_amd.so"
 BUILD = HERE / "_build"
 ARCH = os.environ.get("AMT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["abi.cpp", "fft.hip", "fft_fast.hip", "gemm.hip", "dav.hip", "mlp.hip"]
+SOURCES = ["abi.cpp", "fft.hip", "fft_fast.hip", "gemm.hip", "dav.hip", "mlp.hip", "eig.hip"]
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed-FP32 instructions are disabled on purpose: plain float code only.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -42,7 +43,7 @@ def build(verbose: bool = False) -> Path:
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as pool:
         objs = list(pool.map(_compile, SOURCES))
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs), *LIBS]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")

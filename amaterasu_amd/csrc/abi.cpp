@@ -534,3 +534,14 @@ int amt_adam(float* theta, const float* grad, float* m, float* v, int n, float l
   }
   return 0;
 }
+
+// ------------------------------------------------------------------ subspace eigensolve
+size_t amt_heev_workspace(int n) { return n > 0 ? heev_workspace_bytes(n) : 0; }
+
+int amt_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void* ws, size_t ws_bytes, void* stream) {
+  if (!a || !w || !zt || !ws) {
+    set_error("heev: null argument");
+    return 1;
+  }
+  return launch_heev_lowest(n, a, nev, w, zt, ws, ws_bytes, (hipStream_t)stream);
+}

@@ -82,6 +82,10 @@ int launch_zgemm(int prec, int op, int m, int n, int k, double alpha, const void
                  long long sc, int batch, void* ws, size_t ws_bytes, hipStream_t s);
 size_t zgemm_workspace_bytes(int prec, int op, int m, int n, int k, int batch);
 
+// Lowest eigenpairs of a dense Hermitian FP64 matrix (eig.hip).
+size_t heev_workspace_bytes(int n);
+int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void* ws, size_t ws_bytes, hipStream_t s);
+
 // Learned starting density (mlp.hip).
 int mlp_param_count();
 int mlp_train_waves(long long npts);

@@ -291,11 +291,25 @@ def davidson(apply_hamiltonian, guess, kinetic, *, tolerance: float = 1e-8, max_
     return DavidsonResult(eps, vec, norms, iteration, converged)
 
 
+#: Projected matrices at least this large go to the library's own Hermitian eigensolver
+#: (device tridiagonalisation); smaller ones to torch.linalg.eigh.
+HEEV_MIN = 96
+HEEV_MAX = 1024
+
+
+def _lowest_eigenpairs(proj: torch.Tensor, nbands: int):
+    """(w[:nbands], rows of eigenvectors) of a Hermitian FP64 matrix."""
+    m = proj.shape[0]
+    if HEEV_MIN <= m <= HEEV_MAX:
+        return ops.heev_lowest(proj.contiguous(), nbands)
+    w, vec = torch.linalg.eigh(proj)
+    return w[:nbands], vec[:, :nbands].T.contiguous()
+
+
 def _ritz(proj: torch.Tensor, basis: torch.Tensor, h_basis: torch.Tensor, nbands: int):
     """Lowest ``nbands`` eigenpairs of the projected matrix, rotated into the basis."""
-    w, vec = torch.linalg.eigh(proj)
-    rot = vec[:, :nbands].T.contiguous()
-    return w[:nbands], _rotate(rot, basis), _rotate(rot, h_basis)
+    w, rot = _lowest_eigenpairs(proj, nbands)
+    return w, _rotate(rot, basis), _rotate(rot, h_basis)
 
 
 def solve(name, apply_hamiltonian, guess, kinetic, *, apply_overlap=None, **kwargs) -> DavidsonResult:

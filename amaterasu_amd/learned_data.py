@@ -82,7 +82,9 @@ def _load(dirs, first: int, count: int):
             f = Path(d) / f"cell{c:04d}.npz"
             if f.exists():
                 with np.load(f, allow_pickle=False) as z:
-                    rows.append({k: z[k] for k in z.files})
+                    row = {k: z[k] for k in z.files}
+                row["correction"] = row["correction"].astype(np.float32)
+                rows.append(row)
                 break
         else:
             raise FileNotFoundError(f"cell {c} not found in {dirs}")

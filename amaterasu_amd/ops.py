@@ -151,3 +151,22 @@ def precondition(res: torch.Tensor, v: torch.Tensor, kinetic: torch.Tensor, rows
     check(lib().amt_precondition(prec, res.data_ptr(), v.data_ptr(), kin.data_ptr(), r.data_ptr(),
                                  out.data_ptr(), nout, v.shape[1], stream_handle()), "precondition")
     return out
+
+
+def heev_lowest(a: torch.Tensor, nev: int):
+    """Lowest ``nev`` eigenpairs of a Hermitian complex128 matrix on the device: (w, zt) with
+    w ascending (float64) and zt[k] the k-th eigenvector (row). See amt_heev_lowest."""
+    _require(a, "a")
+    if a.dtype != torch.complex128 or a.dim() != 2 or a.shape[0] != a.shape[1]:
+        raise AmtError("heev: a must be a square complex128 device matrix")
+    n = a.shape[0]
+    if not 1 <= nev <= n:
+        raise AmtError(f"heev: nev={nev} outside 1..{n}")
+    w = torch.empty(nev, dtype=torch.float64, device=a.device)
+    zt = torch.empty((nev, n), dtype=torch.complex128, device=a.device)
+    nbytes = int(lib().amt_heev_workspace(n))
+    ws = workspace(nbytes, "heev")
+    check(lib().amt_heev_lowest(n, a.data_ptr(), nev, w.data_ptr(), zt.data_ptr(), ws.data_ptr(), nbytes,
+                                stream_handle()), "heev")
+    return w, zt
+

@@ -125,6 +125,17 @@ int amt_kb_dij(int precision, const void* proj, const void* dij, const int* blk_
                void* out, int nrow, int nproj, long long sp, long long sd, long long so, int batch, void* stream);
 
 /*
+ * Lowest nev eigenpairs of a dense Hermitian n x n complex128 matrix (row-major, 2 <= n <= 1024):
+ * the subspace eigensolve of rayleigh_ritz (scf/eigen/subspace.py:74, scipy.linalg.eigh there).
+ * w: nev eigenvalues ascending (float64, device); zt: nev x n, row k = eigenvector k (device).
+ * Householder tridiagonalisation on the device (one launch per column), rocSOLVER's
+ * tridiagonal divide and conquer, back-transformation on the device. a is not modified;
+ * ws must hold amt_heev_workspace(n) bytes.
+ */
+size_t amt_heev_workspace(int n);
+int amt_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void* ws, size_t ws_bytes, void* stream);
+
+/*
  * Learned starting density. No counterpart in the reference: it produces the density=
  * argument of run_scf (scf/driver.py:267-299) as a correction over the atomic superposition
  * of PlaneWaveSystem.initial_density (hamiltonian/system.py:147-190), which stays the

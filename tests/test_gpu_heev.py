@@ -1,0 +1,60 @@
+"""The library's Hermitian eigensolver (device tridiagonalisation, tridiagonal divide and
+conquer, device back-transformation) against LAPACK (numpy.linalg.eigh, FP64) on random
+Hermitian matrices and on matrices with exactly degenerate eigenvalues."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _hermitian(n, seed, degenerate=False):
+    rng = np.random.default_rng(seed)
+    if degenerate:  # eigenvalues 0..n/4 each four times, random unitary basis
+        lam = np.repeat(np.arange(n // 4 + 1, dtype=float), 4)[:n]
+        q, _ = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+        return (q * lam) @ q.conj().T
+    x = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    return (x + x.conj().T) / 2
+
+
+@pytest.mark.parametrize("n,nev", [(2, 1), (3, 3), (17, 5), (160, 40), (333, 100), (640, 160), (1024, 256)])
+def test_heev_matches_lapack(n, nev):
+    from amaterasu_amd import ops
+
+    a = _hermitian(n, n)
+    w, zt = ops.heev_lowest(torch.from_numpy(a).cuda(), nev)
+    w, z = w.cpu().numpy(), zt.cpu().numpy().T
+    ref = np.linalg.eigvalsh(a)[:nev]
+    scale = np.abs(np.linalg.eigvalsh(a)).max()
+    assert np.abs(w - ref).max() < 1e-13 * scale * max(1, n / 100)
+    assert np.abs(a @ z - z * w).max() < 1e-12 * scale * max(1, n / 100)
+    assert np.abs(z.conj().T @ z - np.eye(nev)).max() < 1e-12 * max(1, n / 100)
+
+
+def test_heev_degenerate_clusters():
+    from amaterasu_amd import ops
+
+    a = _hermitian(200, 7, degenerate=True)
+    w, zt = ops.heev_lowest(torch.from_numpy(a).cuda(), 60)
+    w, z = w.cpu().numpy(), zt.cpu().numpy().T
+    assert np.abs(w - np.linalg.eigvalsh(a)[:60]).max() < 1e-12 * 50
+    assert np.abs(a @ z - z * w).max() < 1e-11 * 50
+    assert np.abs(z.conj().T @ z - np.eye(60)).max() < 1e-12
+
+
+def test_heev_leaves_input_and_rejects_bad_sizes():
+    from amaterasu_amd import ops
+    from amaterasu_amd._lib import AmtError
+
+    a = torch.from_numpy(_hermitian(50, 3)).cuda()
+    keep = a.clone()
+    ops.heev_lowest(a, 10)
+    assert torch.equal(a, keep)
+    with pytest.raises(AmtError):
+        ops.heev_lowest(torch.zeros((1100, 1100), dtype=torch.complex128, device="cuda"), 4)
+    with pytest.raises(AmtError):
+        ops.heev_lowest(a, 51)
