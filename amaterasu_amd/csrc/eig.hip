@@ -16,7 +16,9 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <cstdlib>
 #include <mutex>
+#include <vector>
 
 #include "kernels.hpp"
 
@@ -277,6 +279,60 @@ static rocblas_handle blas_handle() {
   return g_handles[dev];
 }
 
+// The n launches of the tridiagonalisation, captured once per (device, workspace, n) into
+// a HIP graph and replayed: the launch sequence is fixed by n and every pointer lives in the
+// caller's workspace, so only the input copy (outside the graph) differs between calls.
+struct TridiagGraph {
+  int dev;
+  const void* ws;
+  int n;
+  hipGraphExec_t exec;
+};
+static std::mutex g_graph_mutex;
+static std::vector<TridiagGraph> g_graphs;
+
+static void enqueue_tridiag(hipStream_t s, int n, double2* A, double2* V, double2* taus, double* d, double* e,
+                            double2* p0, double2* p1) {
+  const int G = n - 1 < 256 ? n - 1 : 256;
+  hipLaunchKernelGGL(k_hetrd_start, dim3(G), dim3(eig::NT), 0, s, A, n, V, taus, d, e, p0);
+  for (int j = 0; j <= n - 2; ++j) {
+    const double2* pin = (j & 1) ? p1 : p0;
+    double2* pout = (j & 1) ? p0 : p1;
+    hipLaunchKernelGGL(k_hetrd_step, dim3(G), dim3(eig::NT), 0, s, A, n, j, V, taus, d, e, pin, pout);
+  }
+}
+
+static hipGraphExec_t tridiag_graph(int n, const void* ws, double2* A, double2* V, double2* taus, double* d,
+                                    double* e, double2* p0, double2* p1) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lock(g_graph_mutex);
+  for (const TridiagGraph& g : g_graphs)
+    if (g.dev == dev && g.ws == ws && g.n == n) return g.exec;
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  if (ok) {
+    enqueue_tridiag(cs, n, A, V, taus, d, e, p0, p1);
+    ok = hipStreamEndCapture(cs, &graph) == hipSuccess && graph != nullptr;
+  }
+  if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipStreamDestroy(cs);
+  if (!ok) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (g_graphs.size() >= 32) {  // bounded cache: drop the oldest
+    (void)hipGraphExecDestroy(g_graphs.front().exec);
+    g_graphs.erase(g_graphs.begin());
+  }
+  g_graphs.push_back({dev, ws, n, exec});
+  return exec;
+}
+
 size_t heev_workspace_bytes(int n) {
   // A copy, V, p (2), taus, S (n x n real), d, e, info
   return (size_t)n * n * 16 * 2 + (size_t)2 * n * 16 + (size_t)n * 16 + (size_t)n * n * 8 + (size_t)2 * n * 8 + 256;
@@ -306,16 +362,15 @@ int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void*
     set_error("heev: copy of the input failed");
     return 1;
   }
-  const int G = n - 1 < 256 ? n - 1 : 256;
-  hipLaunchKernelGGL(k_hetrd_start, dim3(G), dim3(eig::NT), 0, s, A, n, V, taus, d, e, p0);
-  if (hipGetLastError() != hipSuccess) {
-    set_error("heev: tridiagonalisation launch failed");
-    return 1;
-  }
-  for (int j = 0; j <= n - 2; ++j) {
-    const double2* pin = (j & 1) ? p1 : p0;
-    double2* pout = (j & 1) ? p0 : p1;
-    hipLaunchKernelGGL(k_hetrd_step, dim3(G), dim3(eig::NT), 0, s, A, n, j, V, taus, d, e, pin, pout);
+  const char* nog = std::getenv("AMT_HEEV_NO_GRAPH");
+  hipGraphExec_t exec = (nog && nog[0] == '1') ? nullptr : tridiag_graph(n, ws, A, V, taus, d, e, p0, p1);
+  if (exec) {
+    if (hipGraphLaunch(exec, s) != hipSuccess) {
+      set_error("heev: tridiagonalisation graph launch failed");
+      return 1;
+    }
+  } else {
+    enqueue_tridiag(s, n, A, V, taus, d, e, p0, p1);
   }
   if (hipGetLastError() != hipSuccess) {
     set_error("heev: tridiagonalisation launch failed");

@@ -303,15 +303,21 @@ def effective_potential(system, grid_side: GridSide, xc: XCFunctional, density: 
 def initial_wavefunctions(system, plan, dtype, seed=12345):
     """``driver.py:255-264``: the reference's default_rng draws (seed + 991 spin + ik), damped by
     1/(1+T), orthonormalised on the device, in the padded layout [spin][k][band][npw_pad]."""
-    wf = torch.zeros((system.nspin, plan.nk, system.nbands, plan.npw_pad), dtype=dtype, device="cuda")
+    extra = getattr(system, "extra", None)
+    key = ("_initial_wavefunctions", seed, system.nspin, plan.nk, system.nbands, plan.npw_pad)
+    if isinstance(extra, dict) and key in extra:  # the same seeded draws, already orthonormalised
+        return extra[key].to(dtype, copy=True)
+    wf = torch.zeros((system.nspin, plan.nk, system.nbands, plan.npw_pad), dtype=torch.complex128, device="cuda")
     for sp in range(system.nspin):
         for ik, s in enumerate(system.wave_spheres):
             n = len(s.g2)
             rng = np.random.default_rng(seed + 991 * sp + ik)
             b = rng.normal(size=(system.nbands, n)) + 1j * rng.normal(size=(system.nbands, n))
             b = b / (1.0 + 0.5 * np.asarray(s.g2))
-            wf[sp, ik, :, :n] = orthonormalize(torch.from_numpy(b).to("cuda")).to(dtype)
-    return wf
+            wf[sp, ik, :, :n] = orthonormalize(torch.from_numpy(b).to("cuda"))
+    if isinstance(extra, dict):
+        extra[key] = wf
+    return wf.to(dtype, copy=True)
 
 
 def run_scf(system, settings: SCFSettings, *, density=None, wavefunctions=None, log=None) -> SCFResult:

@@ -58,3 +58,17 @@ def test_heev_leaves_input_and_rejects_bad_sizes():
         ops.heev_lowest(torch.zeros((1100, 1100), dtype=torch.complex128, device="cuda"), 4)
     with pytest.raises(AmtError):
         ops.heev_lowest(a, 51)
+
+
+def test_heev_repeated_calls_replay_correctly():
+    """The tridiagonalisation is replayed from a cached graph: consecutive calls of one size
+    on different matrices (with other sizes in between) must each match LAPACK."""
+    from amaterasu_amd import ops
+
+    for seed in (11, 12, 13):
+        for n in (200, 120):
+            a = _hermitian(n, seed * n)
+            w, zt = ops.heev_lowest(torch.from_numpy(a).cuda(), 30)
+            assert np.abs(w.cpu().numpy() - np.linalg.eigvalsh(a)[:30]).max() < 1e-12 * n
+            z = zt.cpu().numpy().T
+            assert np.abs(a @ z - z * w.cpu().numpy()).max() < 1e-11 * n
