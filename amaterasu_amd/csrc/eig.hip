@@ -134,23 +134,44 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_start(const double2* __restri
 
 // Launch j (0 <= j <= n-2): finish step j, start step j+1. Trailing numbering of step j:
 // index t <-> row/column j+1+t. p_in: p_j (length n-j-1); p_out: p_{j+1}.
+// Each workgroup owns at most one trailing row per wave (n - 2 <= 4 G, checked by the host):
+// the wave keeps that row in registers from its update to its product with v_{j+1}, and the
+// old pivot row j+1 is fetched into registers at the start, so the only global round trips
+// on the critical path are the loads of v_j, p_j and the owned row.
+constexpr int kRowRegs = eig::MAXN / 64;  // elements of one row per lane
+
 __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A, int n, int j,
                                                         double2* __restrict__ V, double2* __restrict__ taus,
                                                         double* __restrict__ d, double* __restrict__ e,
                                                         const double2* __restrict__ p_in, double2* __restrict__ p_out) {
   __shared__ HetrdShared sh;
   const int len = n - j - 1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the owned row of this wave (if any) and the old pivot row, straight into registers
+  const int myrow = j + 2 + blockIdx.x + wave * gridDim.x;
+  const bool own = myrow < n;
+  double2 rowv[kRowRegs];
+#pragma unroll
+  for (int q = 0; q < kRowRegs; ++q) {
+    const int c = j + 1 + lane + 64 * q;
+    rowv[q] = (own && c < n) ? A[(long long)myrow * n + c] : make_double2(0.0, 0.0);
+  }
+  double2 piv[kRowRegs / 4];  // pivot row j+1, columns j+1 + threadIdx.x + 256 q
+#pragma unroll
+  for (int q = 0; q < kRowRegs / 4; ++q) {
+    const int c = j + 1 + threadIdx.x + eig::NT * q;
+    piv[q] = c < n ? A[(long long)(j + 1) * n + c] : make_double2(0.0, 0.0);
+  }
   const double2 tau = taus[j];
-  // v_j and p_j into LDS; K = -(tau/2) p^H v; w = p + K v
   for (int t = threadIdx.x; t < len; t += eig::NT) {
     sh.v[t] = V[(long long)j * n + t];
     sh.w[t] = p_in[t];
   }
   __syncthreads();
+  // K = -(tau/2) p^H v; w = p + K v
   double2 part = make_double2(0.0, 0.0);
   for (int t = threadIdx.x; t < len; t += eig::NT) {
-    const double2 pc = zconj(sh.w[t]);
-    const double2 pv = zmul(pc, sh.v[t]);
+    const double2 pv = zmul(zconj(sh.w[t]), sh.v[t]);
     part.x += pv.x;
     part.y += pv.y;
   }
@@ -161,46 +182,47 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A,
     sh.w[t] = make_double2(sh.w[t].x + kv.x, sh.w[t].y + kv.y);
   }
   __syncthreads();
-  // update own rows r >= j+2 (trailing t >= 1), columns c >= j+1:  A -= v w^H + w v^H
-  for (int r = j + 2 + blockIdx.x; r < n; r += gridDim.x) {
-    const int tr = r - j - 1;
+  // update the owned row in registers and store it:  A -= v w^H + w v^H
+  if (own) {
+    const int tr = myrow - j - 1;
     const double2 vr = sh.v[tr], wr = sh.w[tr];
-    double2* row = A + (long long)r * n;
-    for (int c = j + 1 + threadIdx.x; c < n; c += eig::NT) {
-      const int tc = c - j - 1;
-      const double2 a = zmul(vr, zconj(sh.w[tc]));
-      const double2 b = zmul(wr, zconj(sh.v[tc]));
-      double2 x = row[c];
-      x.x -= a.x + b.x;
-      x.y -= a.y + b.y;
-      row[c] = x;
+    double2* row = A + (long long)myrow * n;
+#pragma unroll
+    for (int q = 0; q < kRowRegs; ++q) {
+      const int c = j + 1 + lane + 64 * q;
+      if (c < n) {
+        const int tc = c - j - 1;
+        const double2 a = zmul(vr, zconj(sh.w[tc]));
+        const double2 b = zmul(wr, zconj(sh.v[tc]));
+        rowv[q].x -= a.x + b.x;
+        rowv[q].y -= a.y + b.y;
+        row[c] = rowv[q];
+      }
     }
   }
-  // the updated pivot row j+1 (trailing t = 0, never written in this launch) in LDS
   const int len2 = n - j - 2;  // length of column j+1 below the diagonal
   if (len2 <= 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-      const double2 x = A[(long long)(j + 1) * n + (j + 1)];
       const double2 a = zmul(sh.v[0], zconj(sh.w[0]));
-      d[j + 1] = x.x - 2.0 * a.x;
+      d[j + 1] = piv[0].x - 2.0 * a.x;
     }
     return;
   }
-  // new row j+1, columns c >= j+1: old - (v_0 conj(w_c) + w_0 conj(v_c)), v_0 = 1
-  __syncthreads();
-  double2* nr = sh.nr;
-  for (int c = j + 1 + threadIdx.x; c < n; c += eig::NT) {
-    const int tc = c - j - 1;
-    const double2 x = A[(long long)(j + 1) * n + c];
-    const double2 a = zmul(sh.v[0], zconj(sh.w[tc]));
-    const double2 b = zmul(sh.w[0], zconj(sh.v[tc]));
-    nr[tc] = make_double2(x.x - a.x - b.x, x.y - a.y - b.y);
+  // the updated pivot row j+1: old - (v_0 conj(w_c) + w_0 conj(v_c)), v_0 = 1
+#pragma unroll
+  for (int q = 0; q < kRowRegs / 4; ++q) {
+    const int c = j + 1 + threadIdx.x + eig::NT * q;
+    if (c < n) {
+      const int tc = c - j - 1;
+      const double2 a = zmul(sh.v[0], zconj(sh.w[tc]));
+      const double2 b = zmul(sh.w[0], zconj(sh.v[tc]));
+      sh.nr[tc] = make_double2(piv[q].x - a.x - b.x, piv[q].y - a.y - b.y);
+    }
   }
   __syncthreads();
-  if (blockIdx.x == 0 && threadIdx.x == 0) d[j + 1] = nr[0].x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) d[j + 1] = sh.nr[0].x;
   // column j+1 below the diagonal = conj(new row j+1, c >= j+2) -> reflector v_{j+1} (in sh.v)
-  __syncthreads();  // every thread is done with v_j / w_j except row update reads above (finished)
-  for (int t = threadIdx.x; t < len2; t += eig::NT) sh.v[t] = zconj(nr[1 + t]);
+  for (int t = threadIdx.x; t < len2; t += eig::NT) sh.v[t] = zconj(sh.nr[1 + t]);
   __syncthreads();
   double beta;
   double2 tau2;
@@ -213,26 +235,23 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A,
       e[j + 1] = beta;
     }
   }
-  // p_{j+1} for own rows r >= j+2: tau2 * sum_{c >= j+2} A[r][c] v_{j+1}[c - j - 2]
-  // (own rows were updated above by this workgroup)
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int k = 0;
-  for (int r = j + 2 + blockIdx.x; r < n; r += gridDim.x, ++k) {
-    if ((k & 3) != wave) continue;  // rows of this workgroup spread over its 4 waves
-    const double2* row = A + (long long)r * n;
+  // p_{j+1} for the owned row: tau2 * sum_{c >= j+2} A[r][c] v_{j+1}[c - j - 2], from registers
+  if (own) {
     double2 acc = make_double2(0.0, 0.0);
-    for (int c = j + 2 + lane; c < n; c += 64) {
-      const double2 a = row[c];
-      const double2 vv = c == j + 2 ? make_double2(1.0, 0.0) : sh.v[c - j - 2];
-      acc.x += a.x * vv.x - a.y * vv.y;
-      acc.y += a.x * vv.y + a.y * vv.x;
+#pragma unroll
+    for (int q = 0; q < kRowRegs; ++q) {
+      const int c = j + 1 + lane + 64 * q;
+      if (c >= j + 2 && c < n) {
+        const double2 vv = c == j + 2 ? make_double2(1.0, 0.0) : sh.v[c - j - 2];
+        acc.x += rowv[q].x * vv.x - rowv[q].y * vv.y;
+        acc.y += rowv[q].x * vv.y + rowv[q].y * vv.x;
+      }
     }
     for (int off = 32; off > 0; off >>= 1) {
       acc.x += __shfl_down(acc.x, off, 64);
       acc.y += __shfl_down(acc.y, off, 64);
     }
-    if (lane == 0) p_out[r - j - 2] = zmul(tau2, acc);
+    if (lane == 0) p_out[myrow - j - 2] = zmul(tau2, acc);
   }
 }
 
@@ -291,9 +310,17 @@ struct TridiagGraph {
 static std::mutex g_graph_mutex;
 static std::vector<TridiagGraph> g_graphs;
 
+// Workgroups of the tridiagonalisation: one trailing row per wave at most (n - 2 <= 4 G).
+static int hetrd_groups(int n) {
+  int g = (n - 2 + 3) / 4;
+  if (g < 1) g = 1;
+  if (g > 256) g = 256;
+  return g;
+}
+
 static void enqueue_tridiag(hipStream_t s, int n, double2* A, double2* V, double2* taus, double* d, double* e,
                             double2* p0, double2* p1) {
-  const int G = n - 1 < 256 ? n - 1 : 256;
+  const int G = hetrd_groups(n);
   hipLaunchKernelGGL(k_hetrd_start, dim3(G), dim3(eig::NT), 0, s, A, n, V, taus, d, e, p0);
   for (int j = 0; j <= n - 2; ++j) {
     const double2* pin = (j & 1) ? p1 : p0;
