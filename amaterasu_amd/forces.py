@@ -158,6 +158,20 @@ def core_forces(system, density) -> np.ndarray:
     return _real(out, "core force")
 
 
+def padded_block(wavefunctions, spin: int, ik: int, rows, n: int, npw_pad: int) -> torch.Tensor:
+    """Bands ``rows`` of (spin, k) as a zero-padded FP64 device block (len(rows), npw_pad), from
+    the nested ``[spin][k]`` list (numpy or device) or the padded [nspin][nk][nb][npw_pad] block."""
+    if isinstance(wavefunctions, torch.Tensor):
+        blk = wavefunctions[spin, ik][torch.as_tensor(rows, device=wavefunctions.device)]
+    else:
+        b = wavefunctions[spin][ik]
+        b = b if isinstance(b, torch.Tensor) else torch.from_numpy(np.asarray(b))
+        blk = b[torch.as_tensor(rows, device=b.device)]
+    psi = torch.zeros((len(rows), npw_pad), dtype=torch.complex128, device="cuda")
+    psi[:, :n] = blk[:, :n].to("cuda", torch.complex128)
+    return psi
+
+
 def nonlocal_forces(system, wavefunctions, occupations) -> np.ndarray:
     plan = device_plan(system)
     kb = plan.kb_tables(1)
@@ -178,14 +192,7 @@ def nonlocal_forces(system, wavefunctions, occupations) -> np.ndarray:
             act = np.flatnonzero(w > 1e-12)
             if act.size == 0:
                 continue
-            if isinstance(wavefunctions, torch.Tensor):
-                blk = wavefunctions[spin, ik][torch.as_tensor(act, device=wavefunctions.device)]
-            else:
-                b = wavefunctions[spin][ik]
-                b = b if isinstance(b, torch.Tensor) else torch.from_numpy(np.asarray(b))
-                blk = b[torch.as_tensor(act, device=b.device)]
-            psi = torch.zeros((len(act), plan.npw_pad), dtype=torch.complex128, device="cuda")
-            psi[:, :n] = blk[:, :n].to("cuda", torch.complex128)
+            psi = padded_block(wavefunctions, spin, ik, act, n, plan.npw_pad)
             beta = mat[ik]
             p = ops.zgemm(psi, beta, 1)  # [nb][nproj] = psi beta^H
             coupled = p @ dij[ik].to(torch.complex128)

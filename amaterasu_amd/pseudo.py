@@ -147,6 +147,60 @@ def radial_form_factor(ds: Dataset, channel: Channel, q: np.ndarray) -> np.ndarr
     return ds.integrate((ds.r * channel.values)[None, :] * spherical_jn(channel.l, q[:, None] * ds.r[None, :]))
 
 
+def radial_form_factor_derivative(ds: Dataset, channel: Channel, q: np.ndarray) -> np.ndarray:
+    """f_l'(q) = int r^3 beta_l(r) j_l'(q r) dr, the Bessel function differentiated under the
+    integral (``pseudo/formfactors.py:133-157``)."""
+    q = np.atleast_1d(np.asarray(q, dtype=float))
+    x = q[:, None] * ds.r[None, :]
+    return ds.integrate((ds.r**2 * channel.values)[None, :] * spherical_jn(channel.l, x, derivative=True))
+
+
+#: Real solid harmonics C_lm(q) = |q|^l Y_lm(q_hat) as sums of monomials
+#: coefficient * x^a y^b z^c, listed as (coefficient, a, b, c), m = -l..l in the order of
+#: ``_ylm`` (``pseudo/kb.py:98-139``).
+def _solid_monomials():
+    pi = np.pi
+    k1 = np.sqrt(0.75 / pi)
+    k2, k20, k22 = 0.5 * np.sqrt(15.0 / pi), 0.25 * np.sqrt(5.0 / pi), 0.25 * np.sqrt(15.0 / pi)
+    k33, k32, k31 = 0.25 * np.sqrt(17.5 / pi), 0.5 * np.sqrt(105.0 / pi), 0.25 * np.sqrt(10.5 / pi)
+    k30, k32c = 0.25 * np.sqrt(7.0 / pi), 0.25 * np.sqrt(105.0 / pi)
+    return {
+        0: [[(0.5 / np.sqrt(pi), 0, 0, 0)]],
+        1: [[(k1, 0, 1, 0)], [(k1, 0, 0, 1)], [(k1, 1, 0, 0)]],
+        2: [[(k2, 1, 1, 0)], [(k2, 0, 1, 1)], [(2 * k20, 0, 0, 2), (-k20, 2, 0, 0), (-k20, 0, 2, 0)],
+            [(k2, 1, 0, 1)], [(k22, 2, 0, 0), (-k22, 0, 2, 0)]],
+        3: [[(3 * k33, 2, 1, 0), (-k33, 0, 3, 0)], [(k32, 1, 1, 1)],
+            [(4 * k31, 0, 1, 2), (-k31, 2, 1, 0), (-k31, 0, 3, 0)],
+            [(2 * k30, 0, 0, 3), (-3 * k30, 2, 0, 1), (-3 * k30, 0, 2, 1)],
+            [(4 * k31, 1, 0, 2), (-k31, 3, 0, 0), (-k31, 1, 2, 0)],
+            [(k32c, 2, 0, 1), (-k32c, 0, 2, 1)], [(k33, 3, 0, 0), (-3 * k33, 1, 2, 0)]],
+    }
+
+
+_SOLID = _solid_monomials()
+
+
+def _solid_gradient(l: int, v: torch.Tensor) -> torch.Tensor:
+    """dC_lm/dq_alpha of the vectors v (n, 3), shape (2l+1, n, 3): each monomial
+    differentiated term by term (``pseudo/kb.py:142-197``)."""
+    if not 0 <= l <= MAX_L:
+        raise NotImplementedError(f"solid harmonics are implemented up to l={MAX_L}")
+    out = torch.zeros((2 * l + 1, v.shape[0], 3), dtype=v.dtype, device=v.device)
+    powers = [[torch.ones_like(v[:, a])] for a in range(3)]
+    for a in range(3):
+        for _ in range(l):
+            powers[a].append(powers[a][-1] * v[:, a])
+    for m, terms in enumerate(_SOLID[l]):
+        for coef, *e in terms:
+            for axis in range(3):
+                if e[axis] == 0:
+                    continue
+                f = [e[0], e[1], e[2]]
+                f[axis] -= 1
+                out[m, :, axis] += (coef * e[axis]) * powers[0][f[0]] * powers[1][f[1]] * powers[2][f[2]]
+    return out
+
+
 @dataclass
 class Projectors:
     """``KBProjectors``-compatible: ``matrix`` (nproj, npw) complex, ``dij``, ``labels``.

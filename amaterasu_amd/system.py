@@ -133,6 +133,36 @@ def density_form_factor(ds, values, q):
     return ds.integrate(np.asarray(values)[None, :] * spherical_jn(0, q[:, None] * ds.r[None, :]))
 
 
+def local_form_factor_derivative(ds, q):
+    """d v_loc/dq of both pieces of the erf split, zero at q = 0 where v_loc depends on q^2
+    only (``pseudo/formfactors.py:160-185``). With s(r) = r V_loc + Z erf(r):
+    d/dq [(4 pi/q) int s sin(qr) dr] = -(4 pi/q^2) int s sin + (4 pi/q) int s r cos(qr) dr,
+    d/dq [-4 pi Z e^{-q^2/4}/q^2] = 4 pi Z e^{-q^2/4} (1/(2q) + 2/q^3)."""
+    from scipy.special import erf
+
+    q = np.atleast_1d(np.asarray(q, dtype=float))
+    r, z = ds.r, float(ds.valence_charge)
+    out = np.zeros_like(q)
+    nz = q >= 1e-12
+    if np.any(nz):
+        qq = q[nz]
+        scr = r * ds.local_potential + z * erf(r)
+        x = qq[:, None] * r[None, :]
+        out[nz] = (-4.0 * np.pi / qq**2 * ds.integrate(scr[None, :] * np.sin(x))
+                   + 4.0 * np.pi / qq * ds.integrate((scr * r)[None, :] * np.cos(x))
+                   + 4.0 * np.pi * z * np.exp(-0.25 * qq**2) * (0.5 / qq + 2.0 / qq**3))
+    return out
+
+
+def density_form_factor_derivative(ds, values, q):
+    """d/dq int rho(r) j_0(q r) dr = int rho r j_0'(q r) dr (``formfactors.py:133-150``)."""
+    from scipy.special import spherical_jn
+
+    q = np.atleast_1d(np.asarray(q, dtype=float))
+    return ds.integrate((np.asarray(values) * ds.r)[None, :] * spherical_jn(0, q[:, None] * ds.r[None, :],
+                                                                             derivative=True))
+
+
 def superpose(grid, ecut_rho, positions, symbols, radial_of_symbol):
     """F(r) = sum_G [(1/Omega) sum_a f_a(|G|) e^{-i G.tau_a}] e^{iG.r} over the density sphere,
     assembled on the device (``hamiltonian/local.py:42-55``); returns the real part (FP64)."""
@@ -148,7 +178,7 @@ def superpose(grid, ecut_rho, positions, symbols, radial_of_symbol):
     pos = torch.from_numpy(np.asarray(positions, dtype=float)).to("cuda")
     radial = {s: torch.from_numpy(_radial_on_shells(f, sphere.g2)).to("cuda") for s, f in radial_of_symbol.items()}
     for atom, sym in enumerate(symbols):
-        coef += radial[sym] * torch.exp(-1j * (cart @ pos[atom]))
+        coef += radial[sym] * torch.exp(-1j * (cart * pos[atom]).sum(1))
     coef /= grid.volume
     shape = np.array(grid.shape)
     flat = torch.from_numpy(np.ravel_multi_index(tuple((sphere.miller % shape).T), tuple(shape))).to("cuda")

@@ -101,3 +101,38 @@ def si64_scf_system(nbands: int = 160, ecut: float = 20.0, seed: int = 64, xc: s
 
 
 __all__ = ["KBatchSpec", "si64_scf_system", "si8_kbatch", "si64_gamma", "si_conventional", "smooth_potential", "seeded_coefficients"]
+
+
+def main(argv=None) -> int:
+    """``python -m amaterasu_amd.workloads si64-scf [--precision P] [--learned]``: one Si64
+    Gamma-point SCF to 1e-6 Ha/atom and density residual 1e-6, printing each iteration."""
+    import argparse
+    import time
+
+    ap = argparse.ArgumentParser(prog="python -m amaterasu_amd.workloads")
+    ap.add_argument("workload", choices=["si64-scf"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"])
+    ap.add_argument("--learned", action="store_true", help="start from the trained network's density")
+    ap.add_argument("--max-steps", type=int, default=60)
+    a = ap.parse_args(argv)
+    import torch
+
+    from .learned import DensityModel, learned_density
+    from .scf import HARTREE_IN_EV, SCFSettings, run_scf
+
+    system = si64_scf_system()
+    rho = learned_density(system, DensityModel.load()) if a.learned else system.initial_density()
+    settings = SCFSettings(tol_energy=1e-6 * HARTREE_IN_EV, tol_density=1e-6, max_steps=a.max_steps,
+                           precision=a.precision)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    r = run_scf(system, settings, density=rho, log=lambda line: print(f"{time.perf_counter() - t:8.3f}s {line}",
+                                                                      flush=True))
+    torch.cuda.synchronize()
+    print(f"converged {r.converged} after {r.iterations} iterations, E {r.energies.total:.10f} Ha, "
+          f"{time.perf_counter() - t:.3f} s", flush=True)
+    return 0 if r.converged else 1
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

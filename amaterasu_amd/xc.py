@@ -211,6 +211,13 @@ class XCFunctional:
 
     def evaluate_on_grid(self, density: torch.Tensor, gops: GridOps):
         """(energy, potential (nspin, *grid)) for density (nspin, *grid) FP64 on the device."""
+        energy, pot, _, _ = self.evaluate_on_grid_details(density, gops)
+        return energy, pot
+
+    def evaluate_on_grid_details(self, density: torch.Tensor, gops: GridOps):
+        """(energy, potential, gradients, vsigma): for a GGA also the spin-channel gradients
+        (2, *grid, 3) and vsigma (3, *grid) (up-up, up-down, down-down), which the stress
+        needs (``xc/driver.py:47-64``); ``None`` for an LDA."""
         d = density if density.dim() == 4 else density[None]
         nspin = d.shape[0]
         up, dn = (d[0] / 2, d[0] / 2) if nspin == 1 else (d[0], d[1])
@@ -219,7 +226,7 @@ class XCFunctional:
             eps, vu, vd = _lda(up, dn)
             energy = float((eps * (up + dn)).sum().item()) * dv
             pot = vu[None] if nspin == 1 else torch.stack([vu, vd])
-            return energy, pot
+            return energy, pot, None, None
         gu = gops.gradient(up)
         gd = gu if nspin == 1 else gops.gradient(dn)
         suu = (gu * gu).sum(-1)
@@ -229,9 +236,11 @@ class XCFunctional:
         energy = float((eps * (up + dn)).sum().item()) * dv
         vu = vu - gops.divergence(2.0 * vsuu[..., None] * gu + vsud[..., None] * gd)
         if nspin == 1:
-            return energy, vu[None]
-        vd = vd - gops.divergence(2.0 * vsdd[..., None] * gd + vsud[..., None] * gu)
-        return energy, torch.stack([vu, vd])
+            pot = vu[None]
+        else:
+            vd = vd - gops.divergence(2.0 * vsdd[..., None] * gd + vsud[..., None] * gu)
+            pot = torch.stack([vu, vd])
+        return energy, pot, (gu, gd), (vsuu, vsud, vsdd)
 
 
 __all__ = ["GridOps", "XCFunctional"]

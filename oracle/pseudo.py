@@ -186,6 +186,113 @@ def local_potential_form_factor(pseudo: NCPseudopotential, q) -> np.ndarray:
     return out
 
 
+# ------------------------------------------------------- form factor derivatives
+def radial_transform_derivative(grid: RadialGrid, values, l: int, q) -> np.ndarray:
+    """d/dq int f(r) j_l(qr) dr = int f(r) r j_l'(qr) dr (``formfactors.py:133-143``)."""
+    q = np.atleast_1d(np.asarray(q, dtype=float))
+    values = np.asarray(values, dtype=float)
+    if values.shape != grid.r.shape:
+        raise ValueError(f"values has shape {values.shape}, grid has {grid.r.shape}")
+    x = q[:, None] * grid.r[None, :]
+    return grid.integrate(values * grid.r * spherical_jn(int(l), x, derivative=True))
+
+
+def density_form_factor_derivative(grid: RadialGrid, density, q) -> np.ndarray:
+    """``formfactors.py:146-150``."""
+    return radial_transform_derivative(grid, density, 0, q)
+
+
+def projector_form_factor_derivative(grid: RadialGrid, projector: Projector, q) -> np.ndarray:
+    """``formfactors.py:153-157``."""
+    return radial_transform_derivative(grid, grid.r * projector.values, projector.l, q)
+
+
+def local_potential_form_factor_derivative(pseudo: NCPseudopotential, q) -> np.ndarray:
+    """d v_loc/dq of both pieces of the erf split; 0 at q = 0 (``formfactors.py:160-185``).
+
+    With s(r) = r V_loc + Z erf(r): v(q) = (4 pi/q) int s sin(qr) dr - 4 pi Z e^{-q^2/4}/q^2,
+    differentiated term by term."""
+    q = np.atleast_1d(np.asarray(q, dtype=float))
+    g, z = pseudo.grid, float(pseudo.valence_charge)
+    r = g.r
+    out = np.zeros_like(q)
+    nz = q >= 1e-12
+    if not np.any(nz):
+        return out
+    qq = q[nz]
+    s = r * pseudo.local_potential + z * erf(r)
+    arg = qq[:, None] * r[None, :]
+    i_sin = g.integrate(s * np.sin(arg))
+    i_cos = g.integrate(s * r * np.cos(arg))
+    gauss = np.exp(-0.25 * qq**2)
+    out[nz] = (-4.0 * np.pi / qq**2 * i_sin + 4.0 * np.pi / qq * i_cos
+               + 4.0 * np.pi * z * gauss * (0.5 / qq + 2.0 / qq**3))
+    return out
+
+
+# ------------------------------------------------------------------ solid harmonics
+def _solid_table():
+    """C_lm(q) = |q|^l Y_lm(q_hat) as homogeneous polynomials {(a, b, c): coefficient} of
+    x^a y^b z^c, m = -l..l in the order of :func:`real_spherical_harmonics`
+    (``kb.py:98-139``)."""
+    pi = np.pi
+    c1 = np.sqrt(0.75 / pi)
+    c2, cz2, cd2 = 0.5 * np.sqrt(15.0 / pi), 0.25 * np.sqrt(5.0 / pi), 0.25 * np.sqrt(15.0 / pi)
+    ca, cb, cc = 0.25 * np.sqrt(17.5 / pi), 0.5 * np.sqrt(105.0 / pi), 0.25 * np.sqrt(10.5 / pi)
+    cd, ce = 0.25 * np.sqrt(7.0 / pi), 0.25 * np.sqrt(105.0 / pi)
+    return {
+        0: [{(0, 0, 0): 0.5 / np.sqrt(pi)}],
+        1: [{(0, 1, 0): c1}, {(0, 0, 1): c1}, {(1, 0, 0): c1}],
+        2: [{(1, 1, 0): c2}, {(0, 1, 1): c2}, {(0, 0, 2): 2 * cz2, (2, 0, 0): -cz2, (0, 2, 0): -cz2},
+            {(1, 0, 1): c2}, {(2, 0, 0): cd2, (0, 2, 0): -cd2}],
+        3: [{(2, 1, 0): 3 * ca, (0, 3, 0): -ca},
+            {(1, 1, 1): cb},
+            {(0, 1, 2): 4 * cc, (2, 1, 0): -cc, (0, 3, 0): -cc},
+            {(0, 0, 3): 2 * cd, (2, 0, 1): -3 * cd, (0, 2, 1): -3 * cd},
+            {(1, 0, 2): 4 * cc, (3, 0, 0): -cc, (1, 2, 0): -cc},
+            {(2, 0, 1): ce, (0, 2, 1): -ce},
+            {(3, 0, 0): ca, (1, 2, 0): -3 * ca}],
+    }
+
+
+SOLID_HARMONICS = _solid_table()
+
+
+def _poly(terms: dict, q) -> np.ndarray:
+    out = np.zeros(len(q))
+    for (a, b, c), coef in terms.items():
+        out += coef * q[:, 0] ** a * q[:, 1] ** b * q[:, 2] ** c
+    return out
+
+
+def solid_harmonics(l: int, vectors) -> np.ndarray:
+    """C_lm(q), shape (2l+1, n) (``kb.py:98-139``)."""
+    if not 0 <= l <= MAX_L:
+        raise NotImplementedError(f"solid harmonics are implemented up to l={MAX_L}")
+    q = np.atleast_2d(np.asarray(vectors, dtype=float))
+    return np.stack([_poly(t, q) for t in SOLID_HARMONICS[l]])
+
+
+def solid_harmonics_gradient(l: int, vectors) -> np.ndarray:
+    """dC_lm/dq_alpha, shape (2l+1, n, 3), by differentiating each monomial
+    (``kb.py:142-197``)."""
+    if not 0 <= l <= MAX_L:
+        raise NotImplementedError(f"solid harmonics are implemented up to l={MAX_L}")
+    q = np.atleast_2d(np.asarray(vectors, dtype=float))
+    out = np.zeros((2 * l + 1, len(q), 3))
+    for m, terms in enumerate(SOLID_HARMONICS[l]):
+        for axis in range(3):
+            d = {}
+            for exps, coef in terms.items():
+                if exps[axis] == 0:
+                    continue
+                lowered = list(exps)
+                lowered[axis] -= 1
+                d[tuple(lowered)] = d.get(tuple(lowered), 0.0) + coef * exps[axis]
+            out[m, :, axis] = _poly(d, q)
+    return out
+
+
 # ------------------------------------------------------------------ real harmonics
 def real_spherical_harmonics(l: int, directions) -> np.ndarray:
     """Real Y_lm, m = -l..l, l=1 ordered (y, z, x); zero vector -> +z (``kb.py:52-97``)."""

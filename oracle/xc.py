@@ -196,6 +196,12 @@ class XCFunctional:
 
     def evaluate_on_grid(self, density, grid):
         """Returns ``(energy, potential (nspin, *grid))`` (``driver.py:108-164``)."""
+        energy, pot, _, _ = self.evaluate_on_grid_details(density, grid)
+        return energy, pot
+
+    def evaluate_on_grid_details(self, density, grid):
+        """``(energy, potential, gradients (2, *grid, 3) | None, vsigma (3, *grid) | None)``:
+        the fields of the reference's ``XCResult`` the stress reads (``driver.py:47-64``)."""
         d = np.asarray(density, dtype=float)
         if d.ndim == 3:
             d = d[None]
@@ -222,4 +228,5 @@ class XCFunctional:
                 pot[s] = pot[s] - grid.divergence(flux)
         if nspin == 1:
             pot = pot[:1]
-        return energy, pot
+        vsig_grid = None if vsig is None else vsig.T.reshape(3, *grid.shape)
+        return energy, pot, grads, vsig_grid
