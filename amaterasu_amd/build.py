@@ -19,7 +19,7 @@ BUILD = HERE / "_build"
 ARCH = os.environ.get("AMT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["abi.cpp", "fft.hip", "fft_fast.hip", "gemm.hip", "dav.hip", "mlp.hip", "eig.hip"]
-LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+LIBS: list = []
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed-FP32 instructions are disabled on purpose: plain float code only.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

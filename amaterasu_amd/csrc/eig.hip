@@ -10,12 +10,22 @@
 //    row j+1 in LDS from the untouched global copy, derives v_{j+1}, and multiplies its own
 //    rows by it). The kernel boundary is the only global synchronisation; rows are owned
 //    cyclically (row r by workgroup r % G).
-// 2. The real symmetric tridiagonal eigenproblem by rocSOLVER's divide and conquer (dstedc).
+// 2. The lowest eigenpairs of the real symmetric tridiagonal T (d, e) only:
+//    a. eigenvalues 0..m-1 (m = min(n, nev + kClusterMax)) by multisection: one workgroup
+//       per eigenvalue evaluates 512 Sturm counts per step (the pivmin-guarded LDL^T
+//       recurrence of LAPACK's dstebz), so each step narrows the bracket 513-fold;
+//    b. eigenvectors by inverse iteration with each eigenvalue as its own shift (tridiagonal
+//       LU with partial pivoting, pivots floored at eps ||T||, as LAPACK's dlagtf/dlagts),
+//       three rounds from pseudo-random starts. Eigenvalues closer than kClusterGap ||T||
+//       to a neighbour form a cluster, solved by one workgroup: after every round the
+//       cluster's vectors are orthonormalised together (classical Gram-Schmidt, twice), and
+//       a Rayleigh-Ritz step inside the cluster (cyclic Jacobi on the c x c projection)
+//       makes them eigenvectors, so near-degenerate and degenerate eigenvalues come out
+//       orthogonal with small residuals. Vectors of well-separated eigenvalues are accurate
+//       to about eps ||T|| / gap, below 1e-12 for gaps above the cluster threshold.
 // 3. Back-transformation Z = Q S for the lowest nev eigenvectors: one workgroup per
 //    eigenvector applies the reflectors in reverse order (no global synchronisation).
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
-
+#include <cfloat>
 #include <cstdlib>
 #include <mutex>
 #include <vector>
@@ -27,6 +37,9 @@ namespace amt {
 namespace eig {
 constexpr int NT = 256;
 constexpr int MAXN = 1024;
+constexpr int kClusterMax = 64;      // largest cluster solved together
+constexpr double kClusterGap = 2e-4;  // cluster threshold, relative to the Gershgorin bound of ||T||
+constexpr int kInverseRounds = 3;
 }  // namespace eig
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) { return cmul(a, b); }
@@ -286,18 +299,412 @@ __global__ __launch_bounds__(eig::NT) void k_backtransform(const double2* __rest
   for (int i = threadIdx.x; i < n; i += eig::NT) Zt[(long long)k * n + i] = z[i];
 }
 
-// ------------------------------------------------------------------ host
-static std::mutex g_handle_mutex;
-static rocblas_handle g_handles[64] = {};
 
-static rocblas_handle blas_handle() {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-  std::lock_guard<std::mutex> lock(g_handle_mutex);
-  if (!g_handles[dev] && rocblas_create_handle(&g_handles[dev]) != rocblas_status_success) g_handles[dev] = nullptr;
-  return g_handles[dev];
+// ------------------------------------------------------------ tridiagonal eigenpairs
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
 }
 
+// Eigenvalue k (0-based, ascending) of T for k < m, one workgroup per eigenvalue: each
+// step evaluates the Sturm count at 512 points of the bracket (two independent recurrences
+// per thread), so it narrows the bracket 513-fold. scal[0] = the Gershgorin bound of ||T||,
+// scal[1] = pivmin (written by eigenvalue 0's workgroup).
+__device__ __forceinline__ double recip(double q) {
+  // v_rcp_f64 refined by two Newton steps: a correctly signed reciprocal to about 1 ulp,
+  // a third of the latency of the IEEE division sequence on the recurrence's critical path
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(fma(-q, r, 1.0), r, r);
+  return fma(fma(-q, r, 1.0), r, r);
+}
+
+__global__ __launch_bounds__(eig::NT) void k_tri_bisect(const double* __restrict__ d, const double* __restrict__ e,
+                                                        int n, int m, double* __restrict__ wb,
+                                                        double* __restrict__ scal) {
+  __shared__ double dl[eig::MAXN], e2[eig::MAXN];
+  __shared__ double red[3 * (eig::NT / 64)];
+  __shared__ int first[eig::NT / 64];
+  const int k = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (k >= m) return;
+  double lo = DBL_MAX, hi = -DBL_MAX, emax2 = 0.0;
+  for (int i = tid; i < n; i += eig::NT) {
+    const double di = d[i];
+    const double ea = i > 0 ? fabs(e[i - 1]) : 0.0, eb = i + 1 < n ? fabs(e[i]) : 0.0;
+    dl[i] = di;
+    e2[i] = eb * eb;
+    emax2 = fmax(emax2, eb * eb);
+    lo = fmin(lo, di - ea - eb);
+    hi = fmax(hi, di + ea + eb);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, off, 64));
+    hi = fmax(hi, __shfl_xor(hi, off, 64));
+    emax2 = fmax(emax2, __shfl_xor(emax2, off, 64));
+  }
+  if (lane == 0) {
+    red[3 * wave] = lo;
+    red[3 * wave + 1] = hi;
+    red[3 * wave + 2] = emax2;
+  }
+  __syncthreads();
+  for (int w = 0; w < eig::NT / 64; ++w) {
+    lo = fmin(lo, red[3 * w]);
+    hi = fmax(hi, red[3 * w + 1]);
+    emax2 = fmax(emax2, red[3 * w + 2]);
+  }
+  const double tnorm = fmax(fabs(lo), fabs(hi));
+  const double pivmin = DBL_MIN * fmax(1.0, emax2);
+  const double pad = 2.0 * n * DBL_EPSILON * tnorm + 4.0 * pivmin;
+  lo -= pad;
+  hi += pad;
+  if (k == 0 && tid == 0) {
+    scal[0] = tnorm;
+    scal[1] = pivmin;
+  }
+  constexpr int P = 2 * eig::NT;  // points per step
+  for (int it = 0; it < 32; ++it) {
+    if (hi - lo <= 2.0 * DBL_EPSILON * tnorm + 4.0 * pivmin) break;  // dstebz's default accuracy
+    const double width = hi - lo;
+    const double x0 = lo + width * (double)(2 * tid + 1) / (double)(P + 1);
+    const double x1 = lo + width * (double)(2 * tid + 2) / (double)(P + 1);
+    double q0 = dl[0] - x0, q1 = dl[0] - x1;
+    if (fabs(q0) < pivmin) q0 = -pivmin;
+    if (fabs(q1) < pivmin) q1 = -pivmin;
+    int c0 = q0 < 0.0, c1 = q1 < 0.0;
+    for (int i = 1; i < n; ++i) {
+      const double di = dl[i], ei = e2[i - 1];
+      q0 = (di - x0) - ei * recip(q0);
+      q1 = (di - x1) - ei * recip(q1);
+      if (fabs(q0) < pivmin) q0 = -pivmin;
+      if (fabs(q1) < pivmin) q1 = -pivmin;
+      c0 += q0 < 0.0;
+      c1 += q1 < 0.0;
+    }
+    // the first point (0-based j, x_j = lo + width (j+1)/(P+1)) whose count exceeds k
+    int f = c0 >= k + 1 ? 2 * tid : (c1 >= k + 1 ? 2 * tid + 1 : P);
+    for (int off = 32; off > 0; off >>= 1) f = min(f, __shfl_xor(f, off, 64));
+    __syncthreads();
+    if (lane == 0) first[wave] = f;
+    __syncthreads();
+    for (int w = 0; w < eig::NT / 64; ++w) f = min(f, first[w]);
+    const double nlo = f > 0 ? lo + width * (double)f / (double)(P + 1) : lo;
+    const double nhi = f < P ? lo + width * (double)(f + 1) / (double)(P + 1) : hi;
+    lo = nlo;
+    hi = nhi;
+  }
+  if (tid == 0) wb[k] = 0.5 * (lo + hi);
+}
+
+__device__ __forceinline__ double start_value(unsigned long long key) {
+  key += 0x9E3779B97F4A7C15ull;  // splitmix64
+  key = (key ^ (key >> 30)) * 0xBF58476D1CE4E5B9ull;
+  key = (key ^ (key >> 27)) * 0x94D049BB133111EBull;
+  key ^= key >> 31;
+  return (double)(key >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// Factors of T - lam I in LDS, 5 arrays of n: U row i = (1/u0, u1, u2) on columns i, i+1,
+// i+2 (the reciprocal of the pivot, floored at +-tiny), multiplier l[i], and sw[i] != 0
+// when rows i and i+1 were swapped (partial pivoting, as LAPACK's dlagtf).
+struct TriLU {
+  double rinv[eig::MAXN], u1[eig::MAXN], u2[eig::MAXN], l[eig::MAXN], sw[eig::MAXN];
+};
+
+__device__ void tri_factor(const double* __restrict__ dl, const double* __restrict__ el, int n, double lam,
+                           double tiny, TriLU& f) {
+  double a = dl[0] - lam, b = n > 1 ? el[0] : 0.0;
+  for (int i = 0; i + 1 < n; ++i) {
+    const double s = el[i], dd = dl[i + 1] - lam, g = i + 2 < n ? el[i + 1] : 0.0;
+    double p;
+    if (fabs(s) > fabs(a)) {
+      const double mlt = a / s;
+      p = s;
+      f.u1[i] = dd;
+      f.u2[i] = g;
+      f.l[i] = mlt;
+      f.sw[i] = 1.0;
+      a = b - mlt * dd;
+      b = -mlt * g;
+    } else {
+      const double mlt = a != 0.0 ? s / a : 0.0;
+      p = a;
+      f.u1[i] = b;
+      f.u2[i] = 0.0;
+      f.l[i] = mlt;
+      f.sw[i] = 0.0;
+      a = dd - mlt * b;
+      b = g;
+    }
+    if (fabs(p) < tiny) p = p < 0.0 ? -tiny : tiny;
+    f.rinv[i] = 1.0 / p;
+  }
+  if (fabs(a) < tiny) a = a < 0.0 ? -tiny : tiny;
+  f.rinv[n - 1] = 1.0 / a;
+  f.u1[n - 1] = 0.0;
+  f.u2[n - 1] = 0.0;
+  f.l[n - 1] = 0.0;
+  f.sw[n - 1] = 0.0;
+}
+
+// x <- (T - lam I)^{-1} x with the factors.
+__device__ void tri_solve(const TriLU& f, int n, double* __restrict__ x) {
+  for (int i = 0; i + 1 < n; ++i) {
+    if (f.sw[i] != 0.0) {
+      const double t = x[i];
+      x[i] = x[i + 1];
+      x[i + 1] = t - f.l[i] * x[i + 1];
+    } else {
+      x[i + 1] -= f.l[i] * x[i];
+    }
+  }
+  double x1 = 0.0, x2 = 0.0;  // x[i+1], x[i+2]
+  for (int i = n - 1; i >= 0; --i) {
+    const double v = (x[i] - f.u1[i] * x1 - f.u2[i] * x2) * f.rinv[i];
+    x[i] = v;
+    x2 = x1;
+    x1 = v;
+  }
+}
+
+// One inverse-iteration round for row r < m of S: round 0 draws the pseudo-random start
+// and factors T - w_r I (factors kept in work + 5 n r for the later rounds); every round
+// scales the row by tiny (it is unit-norm from the previous round) and solves once.
+__global__ __launch_bounds__(64) void k_tri_inverse(const double* __restrict__ d, const double* __restrict__ e,
+                                                    int n, int m, int round, const double* __restrict__ wb,
+                                                    const double* __restrict__ scal, double* __restrict__ S,
+                                                    double* __restrict__ work) {
+  __shared__ TriLU f;
+  __shared__ double x[eig::MAXN];
+  __shared__ double dl[eig::MAXN], el[eig::MAXN];
+  const int r = blockIdx.x, tid = threadIdx.x;
+  if (r >= m) return;
+  const double tnorm = scal[0];
+  const double tiny = fmax(DBL_EPSILON * tnorm, DBL_MIN);
+  double* row = S + (long long)r * n;
+  double* g = work + 5LL * n * r;
+  if (round == 0) {
+    for (int i = tid; i < n; i += 64) {
+      dl[i] = d[i];
+      el[i] = i + 1 < n ? e[i] : 0.0;
+      x[i] = tiny * start_value(((unsigned long long)r << 32) ^ (unsigned long long)i);
+    }
+    __syncthreads();
+    if (tid == 0) tri_factor(dl, el, n, wb[r], tiny, f);
+    __syncthreads();
+    for (int i = tid; i < n; i += 64) {
+      g[i] = f.rinv[i];
+      g[n + i] = f.u1[i];
+      g[2 * n + i] = f.u2[i];
+      g[3 * n + i] = f.l[i];
+      g[4 * n + i] = f.sw[i];
+    }
+  } else {
+    for (int i = tid; i < n; i += 64) {
+      f.rinv[i] = g[i];
+      f.u1[i] = g[n + i];
+      f.u2[i] = g[2 * n + i];
+      f.l[i] = g[3 * n + i];
+      f.sw[i] = g[4 * n + i];
+      x[i] = tiny * row[i];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) tri_solve(f, n, x);
+  __syncthreads();
+  for (int i = tid; i < n; i += 64) row[i] = x[i];
+}
+
+struct TriShared {
+  double K[eig::kClusterMax][eig::kClusterMax + 1];
+  double Q[eig::kClusterMax][eig::kClusterMax + 1];
+  double dots[eig::kClusterMax];
+  double red[eig::NT / 64 + 1];
+  int order[eig::kClusterMax];
+};
+
+// Workgroup-wide sum of a double (all threads get it).
+__device__ __forceinline__ double group_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < eig::NT / 64; ++w) s += red[w];
+  return s;
+}
+
+// Orthonormalise row t of X against rows 0..t-1 (classical Gram-Schmidt, twice) and
+// normalise it. Returns false when the row is numerically dependent on the earlier ones.
+__device__ bool orthonormalize_row(double* X, int n, int t, TriShared& sh) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* xt = X + (long long)t * n;
+  double before = 0.0;
+  for (int i = threadIdx.x; i < n; i += eig::NT) before += xt[i] * xt[i];
+  before = group_sum(before, sh.red);
+  for (int rep = 0; rep < 2 && t > 0; ++rep) {
+    for (int j = wave; j < t; j += eig::NT / 64) {
+      const double* xj = X + (long long)j * n;
+      double s = 0.0;
+      for (int i = lane; i < n; i += 64) s += xj[i] * xt[i];
+      s = wave_sum(s);
+      if (lane == 0) sh.dots[j] = s;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += eig::NT) {
+      double v = xt[i];
+      for (int j = 0; j < t; ++j) v -= sh.dots[j] * X[(long long)j * n + i];
+      xt[i] = v;
+    }
+    __syncthreads();
+  }
+  double nrm2 = 0.0;
+  for (int i = threadIdx.x; i < n; i += eig::NT) nrm2 += xt[i] * xt[i];
+  nrm2 = group_sum(nrm2, sh.red);
+  if (!(nrm2 > 1e-24 * before) || !(nrm2 > 0.0)) return false;
+  const double sc = 1.0 / sqrt(nrm2);
+  for (int i = threadIdx.x; i < n; i += eig::NT) xt[i] *= sc;
+  __syncthreads();
+  return true;
+}
+
+// After each inverse-iteration round: workgroup k leads the cluster starting at k (if one
+// does) and orthonormalises its rows; after the last round it also solves the Rayleigh-Ritz
+// problem inside the cluster and writes the eigenvalues w[k] for k < nev.
+__global__ __launch_bounds__(eig::NT) void k_tri_cluster(const double* __restrict__ d, const double* __restrict__ e,
+                                                         int n, int m, int nev, int round, int last,
+                                                         const double* __restrict__ wb,
+                                                         const double* __restrict__ scal, double* __restrict__ w,
+                                                         double* __restrict__ S, double* __restrict__ work) {
+  __shared__ TriShared sh;
+  const int k = blockIdx.x;
+  if (k >= nev) return;
+  const double tnorm = scal[0];
+  const double delta = eig::kClusterGap * tnorm;
+  int s0 = k;
+  while (s0 > 0 && wb[s0] - wb[s0 - 1] < delta) --s0;
+  if ((k - s0) % eig::kClusterMax != 0) return;  // not the first member of a cluster
+  int k1 = k + 1;
+  while (k1 < m && k1 - k < eig::kClusterMax && wb[k1] - wb[k1 - 1] < delta) ++k1;
+  const int c = k1 - k;
+  double* X = S + (long long)k * n;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+
+  for (int t = 0; t < c; ++t) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
+      if (orthonormalize_row(X, n, t, sh)) break;
+      // dependent on the earlier rows: restart this row from fresh pseudo-random values
+      double* xt = X + (long long)t * n;
+      for (int i = tid; i < n; i += eig::NT)
+        xt[i] = start_value(((unsigned long long)(k + t) << 32) ^ (unsigned long long)i ^
+                            ((unsigned long long)(attempt + 1 + 4 * round) << 56));
+      __syncthreads();
+    }
+  }
+  if (!last) return;
+  if (c == 1) {
+    if (tid == 0) w[k] = wb[k];
+    return;
+  }
+  // Rayleigh-Ritz inside the cluster: K = X T X^T (c x c), one entry pair per wave step
+  const int npair = c * (c + 1) / 2;
+  for (int pidx = wave; pidx < npair; pidx += eig::NT / 64) {
+    int a = 0, rem = pidx;
+    while (rem >= c - a) {
+      rem -= c - a;
+      ++a;
+    }
+    const int b = a + rem;
+    const double* xa = X + (long long)a * n;
+    const double* xb = X + (long long)b * n;
+    double s = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      double tb = d[i] * xb[i];
+      if (i > 0) tb += e[i - 1] * xb[i - 1];
+      if (i + 1 < n) tb += e[i] * xb[i + 1];
+      s += xa[i] * tb;
+    }
+    s = wave_sum(s);
+    if (lane == 0) {
+      sh.K[a][b] = s;
+      sh.K[b][a] = s;
+    }
+  }
+  for (int idx = tid; idx < c * c; idx += eig::NT) sh.Q[idx / c][idx % c] = (idx / c == idx % c) ? 1.0 : 0.0;
+  __syncthreads();
+  if (wave == 0) {  // cyclic Jacobi, lane r owns row r of K and Q
+    const double skip = 1e-18 * tnorm;
+    for (int sweep = 0; sweep < 30; ++sweep) {
+      bool rotated = false;
+      for (int p = 0; p < c - 1; ++p) {
+        for (int q = p + 1; q < c; ++q) {
+          const double apq = sh.K[p][q];
+          if (fabs(apq) <= skip) continue;
+          const double app = sh.K[p][p], aqq = sh.K[q][q];
+          const double theta = (aqq - app) / (2.0 * apq);
+          const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+          const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+          const int r = lane;
+          double krp = 0.0, krq = 0.0, qrp = 0.0, qrq = 0.0;
+          if (r < c) {
+            krp = sh.K[r][p];
+            krq = sh.K[r][q];
+            qrp = sh.Q[r][p];
+            qrq = sh.Q[r][q];
+          }
+          __builtin_amdgcn_wave_barrier();
+          if (r < c) {
+            if (r != p && r != q) {
+              const double np_ = cs * krp - sn * krq, nq = sn * krp + cs * krq;
+              sh.K[r][p] = np_;
+              sh.K[p][r] = np_;
+              sh.K[r][q] = nq;
+              sh.K[q][r] = nq;
+            }
+            sh.Q[r][p] = cs * qrp - sn * qrq;
+            sh.Q[r][q] = sn * qrp + cs * qrq;
+          }
+          if (r == 0) {
+            sh.K[p][p] = app - t * apq;
+            sh.K[q][q] = aqq + t * apq;
+            sh.K[p][q] = 0.0;
+            sh.K[q][p] = 0.0;
+          }
+          __builtin_amdgcn_wave_barrier();
+          rotated = true;
+        }
+      }
+      if (!rotated) break;
+    }
+    if (lane == 0) {  // ascending order of the Ritz values
+      for (int t = 0; t < c; ++t) sh.order[t] = t;
+      for (int t = 0; t < c; ++t) {
+        int best = t;
+        for (int u = t + 1; u < c; ++u)
+          if (sh.K[sh.order[u]][sh.order[u]] < sh.K[sh.order[best]][sh.order[best]]) best = u;
+        const int tmp = sh.order[t];
+        sh.order[t] = sh.order[best];
+        sh.order[best] = tmp;
+      }
+    }
+  }
+  __syncthreads();
+  // rotated vectors into the (no longer needed) factor storage of the cluster's rows, then back
+  double* tmp = work + 5LL * n * k;
+  for (int idx = tid; idx < c * n; idx += eig::NT) {
+    const int t = idx / n, i = idx - t * n;
+    const int o = sh.order[t];
+    double v = 0.0;
+    for (int a = 0; a < c; ++a) v += sh.Q[a][o] * X[(long long)a * n + i];
+    tmp[idx] = v;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < c * n; idx += eig::NT) X[idx] = tmp[idx];
+  for (int t = tid; t < c; t += eig::NT)
+    if (k + t < nev) w[k + t] = sh.K[sh.order[t]][sh.order[t]];
+}
+
+// ------------------------------------------------------------------ host
 // The n launches of the tridiagonalisation, captured once per (device, workspace, n) into
 // a HIP graph and replayed: the launch sequence is fixed by n and every pointer lives in the
 // caller's workspace, so only the input copy (outside the graph) differs between calls.
@@ -361,8 +768,9 @@ static hipGraphExec_t tridiag_graph(int n, const void* ws, double2* A, double2* 
 }
 
 size_t heev_workspace_bytes(int n) {
-  // A copy, V, p (2), taus, S (n x n real), d, e, info
-  return (size_t)n * n * 16 * 2 + (size_t)2 * n * 16 + (size_t)n * 16 + (size_t)n * n * 8 + (size_t)2 * n * 8 + 256;
+  // A copy, V, p (2), taus, S (n x n real), d, e, bisection values, 4 scalars, LU factors (5 n x n)
+  return (size_t)n * n * 16 * 2 + (size_t)2 * n * 16 + (size_t)n * 16 + (size_t)n * n * 8 + (size_t)3 * n * 8 + 32 +
+         (size_t)5 * n * n * 8;
 }
 
 int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void* ws, size_t ws_bytes,
@@ -384,7 +792,9 @@ int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void*
   double* S = reinterpret_cast<double*>(taus + n);
   double* d = S + (size_t)n * n;
   double* e = d + n;
-  rocblas_int* info = reinterpret_cast<rocblas_int*>(e + n);
+  double* wb = e + n;
+  double* scal = wb + n;
+  double* lu = scal + 4;
   if (hipMemcpyAsync(A, a, (size_t)n * n * 16, hipMemcpyDeviceToDevice, s) != hipSuccess) {
     set_error("heev: copy of the input failed");
     return 1;
@@ -403,23 +813,21 @@ int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void*
     set_error("heev: tridiagonalisation launch failed");
     return 1;
   }
-  rocblas_handle h = blas_handle();
-  if (!h || rocblas_set_stream(h, s) != rocblas_status_success) {
-    set_error("heev: rocBLAS handle unavailable");
-    return 1;
+  const int m = nev + eig::kClusterMax < n ? nev + eig::kClusterMax : n;
+  hipLaunchKernelGGL(k_tri_bisect, dim3(m), dim3(eig::NT), 0, s, d, e, n, m, wb, scal);
+  for (int round = 0; round < eig::kInverseRounds; ++round) {
+    hipLaunchKernelGGL(k_tri_inverse, dim3(m), dim3(64), 0, s, d, e, n, m, round, wb, scal, S, lu);
+    hipLaunchKernelGGL(k_tri_cluster, dim3(nev), dim3(eig::NT), 0, s, d, e, n, m, nev, round,
+                       round + 1 == eig::kInverseRounds ? 1 : 0, wb, scal, w, S, lu);
   }
-  if (rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, d, e, S, n, info) != rocblas_status_success) {
-    set_error("heev: rocsolver_dstedc failed");
+  if (hipGetLastError() != hipSuccess) {
+    set_error("heev: tridiagonal eigenpair launch failed");
     return 1;
   }
   hipLaunchKernelGGL(k_backtransform, dim3(nev), dim3(eig::NT), 0, s, V, taus, S, n,
                      static_cast<double2*>(zt));
   if (hipGetLastError() != hipSuccess) {
     set_error("heev: back-transformation launch failed");
-    return 1;
-  }
-  if (hipMemcpyAsync(w, d, (size_t)nev * 8, hipMemcpyDeviceToDevice, s) != hipSuccess) {
-    set_error("heev: copy of the eigenvalues failed");
     return 1;
   }
   return 0;

@@ -232,20 +232,38 @@ def occupations(eig: torch.Tensor, weights: torch.Tensor, nelectrons: float, sch
 
 
 class PulayMixer:
-    """``mixing.py:131-201`` with the history in device memory."""
+    """``mixing.py:131-201`` with the history in device memory.
+
+    The residual overlap matrix is kept incrementally: each new residual is dotted with the
+    stored ones in one fused multiply-and-reduce (a history x grid matrix product here is a
+    skinny GEMM the BLAS library handles poorly), and the oldest row and column are dropped
+    with the oldest residual."""
 
     def __init__(self, beta=0.4, history=8, precondition=None, linear=False):
         self.beta, self.history, self.linear = beta, history, linear
         self.precondition = precondition or (lambda r: r)
         self.inputs, self.residuals = [], []
+        self._ov = np.zeros((0, 0))
+
+    def _drop_oldest(self) -> None:
+        del self.inputs[0]
+        del self.residuals[0]
+        self._ov = self._ov[1:, 1:]
 
     def __call__(self, din: torch.Tensor, dout: torch.Tensor) -> torch.Tensor:
         r = dout - din
         self.inputs.append(din.clone())
         self.residuals.append(r)
+        if not self.linear:
+            n = len(self.residuals)
+            flat = torch.stack([x.reshape(-1) for x in self.residuals])
+            dots = (flat * r.reshape(1, -1)).sum(1).cpu().numpy()
+            ov = np.zeros((n, n))
+            ov[: n - 1, : n - 1] = self._ov
+            ov[n - 1, :] = ov[:, n - 1] = dots
+            self._ov = ov
         if len(self.inputs) > self.history:
-            del self.inputs[0]
-            del self.residuals[0]
+            self._drop_oldest()
         if self.linear:
             return din + self.beta * self.precondition(r)
         while True:
@@ -253,8 +271,7 @@ class PulayMixer:
             if n == 1:
                 c = np.ones(1)
                 break
-            flat = torch.stack([x.reshape(-1) for x in self.residuals])
-            ov = (flat @ flat.T).cpu().numpy()
+            ov = self._ov
             scale = float(np.trace(ov)) / n
             m = np.ones((n + 1, n + 1))
             m[:n, :n] = ov / scale
@@ -268,8 +285,7 @@ class PulayMixer:
             if sol is not None and np.all(np.isfinite(sol)) and np.max(np.abs(sol[:n])) < 1e4:
                 c = sol[:n]
                 break
-            del self.inputs[0]
-            del self.residuals[0]
+            self._drop_oldest()
         ct = torch.from_numpy(c).to(din)
         xin = torch.einsum("i,i...->...", ct, torch.stack(self.inputs))
         xr = torch.einsum("i,i...->...", ct, torch.stack(self.residuals))
