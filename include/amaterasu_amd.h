@@ -128,9 +128,10 @@ int amt_kb_dij(int precision, const void* proj, const void* dij, const int* blk_
  * Lowest nev eigenpairs of a dense Hermitian n x n complex128 matrix (row-major, 2 <= n <= 1024):
  * the subspace eigensolve of rayleigh_ritz (scf/eigen/subspace.py:74, scipy.linalg.eigh there).
  * w: nev eigenvalues ascending (float64, device); zt: nev x n, row k = eigenvector k (device).
- * Householder tridiagonalisation on the device (one launch per column), rocSOLVER's
- * tridiagonal divide and conquer, back-transformation on the device. a is not modified;
- * ws must hold amt_heev_workspace(n) bytes.
+ * Householder tridiagonalisation on the device (one launch per column); the lowest
+ * eigenpairs of the tridiagonal by multisection bisection and inverse iteration, with a
+ * Rayleigh-Ritz step inside clusters of close eigenvalues; back-transformation on the
+ * device. a is not modified; ws must hold amt_heev_workspace(n) bytes.
  */
 size_t amt_heev_workspace(int n);
 int amt_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void* ws, size_t ws_bytes, void* stream);
