@@ -37,6 +37,7 @@ SIGNATURES = {
     "amt_gather": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_dbl, _vp]),
     "amt_fft3d": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_dbl, _vp]),
     "amt_zgemm_workspace": (_c_sz, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
+    "amt_zgemm_set_split": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
     "amt_zgemm": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_dbl, _vp, _c_ll, _c_ll, _vp, _c_ll, _c_ll,
                            _c_dbl, _vp, _c_ll, _c_ll, _c_int, _vp, _c_sz, _vp]),
     "amt_residual": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp]),

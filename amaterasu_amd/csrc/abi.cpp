@@ -403,6 +403,14 @@ size_t amt_zgemm_workspace(int precision, int op, int m, int n, int k, int batch
   return zgemm_workspace_bytes(precision, op, m, n, k, batch);
 }
 
+int amt_zgemm_set_split(int precision, int op, int m, int n, int k, int batch, int split) {
+  if ((precision != kFP32 && precision != kFP64) || (op != 0 && op != 1) || m < 0 || n < 0 || k < 0 || batch < 0) {
+    set_error("zgemm_set_split: bad argument");
+    return 1;
+  }
+  return set_zgemm_split(precision, op, m, n, k, batch, split);
+}
+
 int amt_zgemm(int precision, int op, int m, int n, int k, double alpha, const void* a, long long lda, long long sa,
               const void* b, long long ldb, long long sb, double beta, void* c, long long ldc, long long sc,
               int batch, void* ws, size_t ws_bytes, void* stream) {

@@ -18,6 +18,11 @@
 // LDS, stored k-major so fragment reads are unit-stride across lanes. Long-K /
 // small-output products (projections onto a few hundred bands) are split along K
 // into enough workgroups to fill 256 CUs, with a deterministic second-pass reduction.
+#include <array>
+#include <map>
+#include <mutex>
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace amt {
@@ -179,20 +184,53 @@ __global__ void k_splitk_reduce(int M, int N, int nsplit, const typename Cplx<R>
   *dst = o;
 }
 
-static int choose_split(int m, int n, int k, int batch) {
+// Split-K factor: a tuned value for the shape's bucket (tiles of M and N, K in 256s,
+// batch) when one was recorded with set_zgemm_split, else a heuristic that aims at about
+// 1024 workgroups and keeps at least 256 of K per split.
+static std::mutex g_split_mutex;
+static std::map<std::array<int, 6>, int> g_split_table;
+
+static std::array<int, 6> split_key(int prec, int op, int m, int n, int k, int batch) {
+  return {prec, op, (m + BM - 1) / BM, (n + BN - 1) / BN, (k + 255) / 256, batch};
+}
+
+static int choose_split(int prec, int op, int m, int n, int k, int batch) {
+  const long long kmax = (k + 255) / 256;
+  {
+    std::lock_guard<std::mutex> lock(g_split_mutex);
+    auto it = g_split_table.find(split_key(prec, op, m, n, k, batch));
+    if (it != g_split_table.end()) {
+      long long s = it->second;
+      if (s > kmax) s = kmax;
+      return s < 1 ? 1 : (int)s;
+    }
+  }
   const long long tiles = (long long)((m + BM - 1) / BM) * ((n + BN - 1) / BN) * batch;
   if (tiles >= 512) return 1;
   long long want = (1024 + tiles - 1) / tiles;
-  const long long kmax = (k + 255) / 256;  // keep at least 256 of K per split
   if (want > kmax) want = kmax;
   if (want < 1) want = 1;
   if (want > 256) want = 256;
   return (int)want;
 }
 
+int set_zgemm_split(int prec, int op, int m, int n, int k, int batch, int split) {
+  if (split < 0 || split > 256) {
+    set_error("zgemm split must be 0 (heuristic) .. 256");
+    return 1;
+  }
+  std::lock_guard<std::mutex> lock(g_split_mutex);
+  const auto key = split_key(prec, op, m, n, k, batch);
+  if (split == 0)
+    g_split_table.erase(key);
+  else
+    g_split_table[key] = split;
+  return 0;
+}
+
 size_t zgemm_workspace_bytes(int prec, int op, int m, int n, int k, int batch) {
   (void)op;
-  const int ns = choose_split(m, n, k, batch);
+  const int ns = choose_split(prec, op, m, n, k, batch);
   if (ns <= 1) return 0;
   const size_t c = prec == kFP64 ? 16 : 8;
   return (size_t)batch * ns * m * n * c;
@@ -204,7 +242,7 @@ static int zgemm_t(int op, int m, int n, int k, double alpha, const void* a, lon
                    int batch, void* ws, size_t ws_bytes, hipStream_t s) {
   using C = typename Cplx<R>::T;
   if (m <= 0 || n <= 0 || batch <= 0) return 0;
-  int ns = choose_split(m, n, k, batch);
+  int ns = choose_split(std::is_same<R, double>::value ? kFP64 : kFP32, op, m, n, k, batch);
   if (k <= 0) ns = 1;
   const size_t need = ns > 1 ? (size_t)batch * ns * m * n * sizeof(C) : 0;
   C* part = nullptr;

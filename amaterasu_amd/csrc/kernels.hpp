@@ -81,6 +81,7 @@ int launch_zgemm(int prec, int op, int m, int n, int k, double alpha, const void
                  long long sa, const void* b, long long ldb, long long sb, double beta, void* c, long long ldc,
                  long long sc, int batch, void* ws, size_t ws_bytes, hipStream_t s);
 size_t zgemm_workspace_bytes(int prec, int op, int m, int n, int k, int batch);
+int set_zgemm_split(int prec, int op, int m, int n, int k, int batch, int split);
 
 // Lowest eigenpairs of a dense Hermitian FP64 matrix (eig.hip).
 size_t heev_workspace_bytes(int n);

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import torch
 
+from . import autotune
 from ._lib import FP32, FP64, AmtError, check, lib
 
 _WORKSPACES: dict = {}
@@ -96,12 +97,20 @@ def zgemm(a: torch.Tensor, b: torch.Tensor, op: int, *, alpha: float = 1.0, beta
     if tuple(c3.shape) != (nb, m, n):
         raise AmtError(f"zgemm out has shape {tuple(out.shape)}, expected {(nb, m, n)}")
     prec = precision_of(a)
-    wsb = lib().amt_zgemm_workspace(prec, op, m, n, k, nb)
-    ws = workspace(wsb, "gemm") if wsb else None
-    check(lib().amt_zgemm(prec, op, m, n, k, float(alpha), a3.data_ptr(), a3.stride(1), a3.stride(0),
-                          b3.data_ptr(), b3.stride(1), b3.stride(0), float(beta), c3.data_ptr(), c3.stride(1),
-                          c3.stride(0), nb, ws.data_ptr() if ws is not None else None, int(wsb), stream_handle()),
-          "zgemm")
+
+    def launch(dst, al, be):
+        wsb = lib().amt_zgemm_workspace(prec, op, m, n, k, nb)
+        ws = workspace(wsb, "gemm") if wsb else None
+        check(lib().amt_zgemm(prec, op, m, n, k, float(al), a3.data_ptr(), a3.stride(1), a3.stride(0),
+                              b3.data_ptr(), b3.stride(1), b3.stride(0), float(be), dst.data_ptr(), dst.stride(1),
+                              dst.stride(0), nb, ws.data_ptr() if ws is not None else None, int(wsb),
+                              stream_handle()), "zgemm")
+
+    if autotune.enabled() and autotune.bucket(prec, op, m, n, k, nb) not in autotune._TUNED:
+        scratch = torch.empty((nb, m, n), dtype=a.dtype, device=a.device)
+        autotune.tune_zgemm(lambda: launch(scratch, 1.0, 0.0), prec, op, m, n, k, nb)
+        del scratch
+    launch(c3, alpha, beta)
     return out
 
 

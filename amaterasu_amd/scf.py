@@ -80,6 +80,11 @@ class Energies:
     def free_energy(self) -> float:
         return self.total - self.entropy
 
+    @property
+    def extrapolated(self) -> float:
+        """E0 = (E + F)/2, the sigma -> 0 estimate (``driver.py:83-86``)."""
+        return self.total - 0.5 * self.entropy
+
 
 @dataclass
 class SCFResult:
@@ -96,6 +101,8 @@ class SCFResult:
     electron_count: float = 0.0
     seconds: float = 0.0
     davidson_steps: int = 0
+    #: total moment in Bohr magnetons, sum_k w_k sum_n (f_up - f_down) (``occupations.py:236-237``)
+    magnetization: float = 0.0
 
 
 class GridSide:
@@ -446,8 +453,13 @@ def run_scf(system, settings: SCFSettings, *, density=None, wavefunctions=None, 
     wfs = [[wf[sp, ik, :, : plan.npw[ik]].to(torch.complex128).cpu().numpy() for ik in range(nk)]
            for sp in range(nspin)]
     count = float(dout.sum().item()) * dv
-    return SCFResult(conv, it, energies, eig.cpu().numpy(), occ.cpu().numpy(), fermi, gap, dout.cpu().numpy(), wfs,
-                     hist, count, time.perf_counter() - t_start, dav_steps)
+    occ_h = occ.cpu().numpy()
+    magnetization = 0.0
+    if occ_h.shape[0] == 2:
+        per_spin = np.einsum("k,skn->s", np.asarray(system.kpoints.weights, dtype=float), occ_h)
+        magnetization = float(per_spin[0] - per_spin[1])
+    return SCFResult(conv, it, energies, eig.cpu().numpy(), occ_h, fermi, gap, dout.cpu().numpy(), wfs,
+                     hist, count, time.perf_counter() - t_start, dav_steps, magnetization)
 
 
 __all__ = ["Energies", "GridSide", "PulayMixer", "SCFResult", "SCFSettings", "occupations", "run_scf"]

@@ -260,6 +260,7 @@ def attach_ionic_fields(system: DeviceSystem, datasets: dict, ecut_rho: float, x
     valence = superpose(system.grid, ecut_rho, system.positions, syms, vradial).clamp(min=1e-12)
     dv = system.grid.cell_volume_per_point
     valence *= system.nelectrons / (float(valence.sum().item()) * dv)
+    system.extra["initial_magmoms"] = np.zeros(len(syms)) if magmoms is None else np.asarray(magmoms, dtype=float)
     if system.nspin == 1:
         system.extra["initial_density"] = valence[None].contiguous()
     else:

@@ -106,6 +106,13 @@ int amt_fft3d(int precision, int n1, int n2, int n3, int batch, void* data, int 
  * amt_zgemm_workspace(...) bytes (0 when no split along k is used).
  */
 size_t amt_zgemm_workspace(int precision, int op, int m, int n, int k, int batch);
+/*
+ * Tuning hook, no counterpart in the reference: records the split along k that amt_zgemm
+ * (and amt_zgemm_workspace) use for every product in the shape's bucket (precision, op,
+ * ceil(m/64), ceil(n/64), ceil(k/256), batch); split 0 restores the built-in heuristic.
+ * For callers that time the candidates themselves (amaterasu_amd.autotune does).
+ */
+int amt_zgemm_set_split(int precision, int op, int m, int n, int k, int batch, int split);
 int amt_zgemm(int precision, int op, int m, int n, int k, double alpha, const void* a, long long lda, long long sa,
               const void* b, long long ldb, long long sb, double beta, void* c, long long ldc, long long sc,
               int batch, void* ws, size_t ws_bytes, void* stream);
