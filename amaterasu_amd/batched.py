@@ -29,9 +29,7 @@ import numpy as np
 import torch
 
 from . import ops
-from .eigen import orthonormalize
-
-_PAD_EIG = 1.0e6
+from .eigen import _lowest_eigenpairs, orthonormalize
 
 
 @dataclass
@@ -146,12 +144,64 @@ def _gram_rows(block: torch.Tensor, used: torch.Tensor, threshold: float):
     return torch.where(ok[:, None, None], again, v), kept
 
 
+_SOLVE_STREAMS: dict = {}
+
+
+def _solve_streams(count: int) -> list:
+    """Side streams (default priority) for the per-k eigensolves, created once per device."""
+    dev = torch.cuda.current_device()
+    have = _SOLVE_STREAMS.setdefault(dev, [])
+    while len(have) < count:
+        have.append(torch.cuda.Stream())
+    return have[:count]
+
+
+def _lowest(proj: torch.Tensor, used, nbands: int):
+    """Lowest ``nbands`` eigenpairs of the leading used[k] x used[k] block of each projected
+    matrix, at its own size with the per-k Davidson's solver (the library's for 96..2048 rows):
+    eigenvalues (nk, nbands) and eigenvector rows (nk, nbands, n), zero on padded coordinates.
+
+    The solves of different k-points are independent chains of small, latency-bound launches,
+    so they run concurrently, one side stream and one workspace per k, ordered against the
+    caller's stream by events."""
+    nk, n = proj.shape[0], proj.shape[-1]
+    used = [int(u) for u in (used.tolist() if isinstance(used, torch.Tensor) else used)]
+    w = torch.empty((nk, nbands), dtype=torch.float64, device=proj.device)
+    rot = torch.zeros((nk, nbands, n), dtype=proj.dtype, device=proj.device)
+    main = torch.cuda.current_stream()
+    if nk == 1:
+        c = used[0]
+        wk, zk = _lowest_eigenpairs(proj[0, :c, :c], nbands)
+        w[0] = wk
+        rot[0, :, :c] = zk
+        return w, rot
+    ready = torch.cuda.Event()
+    ready.record(main)
+    results = []
+    for k, side in enumerate(_solve_streams(nk)):
+        c = used[k]
+        side.wait_event(ready)
+        with torch.cuda.stream(side):
+            blk = proj[k, :c, :c].contiguous()
+            wk, zk = _lowest_eigenpairs(blk, nbands, workspace_tag=f"heev{k}")
+            done = torch.cuda.Event()
+            done.record(side)
+        proj.record_stream(side)
+        results.append((k, c, wk, zk, done))
+    for k, c, wk, zk, done in results:
+        main.wait_event(done)
+        wk.record_stream(main)
+        zk.record_stream(main)
+        w[k] = wk
+        rot[k, :, :c] = zk
+    return w, rot
+
+
 def _rayleigh_ritz(basis, h_basis, used, nbands):
     m = _bgram(basis, h_basis)  # conj of the projected matrix
     proj = _herm(m.conj())
-    w, vec = _eigh(proj, used, _PAD_EIG)
-    rot = vec[:, :, :nbands].transpose(-1, -2).contiguous()
-    return w[:, :nbands].contiguous(), _brot(rot, basis), _brot(rot, h_basis)
+    w, rot = _lowest(proj, used, nbands)
+    return w, _brot(rot, basis), _brot(rot, h_basis)
 
 
 def solve_batched(apply_block, guess: torch.Tensor, kinetic: torch.Tensor, npw, *, tolerance: float = 1e-8,

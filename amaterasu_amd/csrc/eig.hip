@@ -1,4 +1,4 @@
-// Lowest eigenpairs of a dense Hermitian matrix (FP64 complex, n <= 1024): the subspace
+// Lowest eigenpairs of a dense Hermitian matrix (FP64 complex, n <= 2048): the subspace
 // eigensolve of Rayleigh-Ritz (scf/eigen/subspace.py:74).
 //
 // 1. Householder tridiagonalisation A = Q T Q^H (the conventions of LAPACK's zhetd2,
@@ -36,7 +36,7 @@ namespace amt {
 
 namespace eig {
 constexpr int NT = 256;
-constexpr int MAXN = 1024;
+constexpr int MAXN = 2048;
 constexpr int kClusterMax = 64;      // largest cluster solved together
 constexpr double kClusterGap = 2e-4;  // cluster threshold, relative to the Gershgorin bound of ||T||
 constexpr int kInverseRounds = 3;
@@ -96,21 +96,25 @@ __device__ void larfg(double2* x, int len, double* red, double* beta_out, double
   __syncthreads();
 }
 
-// Shared state of one launch.
+// LDS of one launch: three vectors of n in dynamic LDS (sized to the matrix, so that the
+// solves of several k-points can share the CUs) and the reduction scratch.
 struct HetrdShared {
-  double2 v[eig::MAXN];   // reflector of the current step (v[0] = 1), trailing numbering
-  double2 w[eig::MAXN];   // w of the current step
-  double2 nr[eig::MAXN];  // the updated pivot row of the next step
-  double red[2 * (eig::NT / 64) + 2];
-  double2 tau;
-  double beta;
+  double2* v;   // reflector of the current step (v[0] = 1), trailing numbering
+  double2* w;   // w of the current step
+  double2* nr;  // the updated pivot row of the next step
+  double* red;
 };
+extern __shared__ __attribute__((aligned(16))) double2 hetrd_dyn[];
+extern __shared__ __attribute__((aligned(16))) double tri_dyn[];
+#define HETRD_SHARED(n)                              \
+  __shared__ double hetrd_red[2 * (eig::NT / 64) + 2]; \
+  HetrdShared sh{hetrd_dyn, hetrd_dyn + (n), hetrd_dyn + 2 * (n), hetrd_red}
 
 // Prologue: v_0 from row 0, d_0 / e_0 / tau_0, V[0], and p_0 = tau_0 A[1:,1:] v_0 for own rows.
 __global__ __launch_bounds__(eig::NT) void k_hetrd_start(const double2* __restrict__ A, int n, double2* __restrict__ V,
                                                          double2* __restrict__ taus, double* __restrict__ d,
                                                          double* __restrict__ e, double2* __restrict__ p) {
-  __shared__ HetrdShared sh;
+  HETRD_SHARED(n);
   const int len = n - 1;  // length of column 0 below the diagonal
   for (int t = threadIdx.x; t < len; t += eig::NT) sh.v[t] = zconj(A[1 + t]);  // column 0 = conj(row 0)
   __syncthreads();
@@ -157,7 +161,7 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A,
                                                         double2* __restrict__ V, double2* __restrict__ taus,
                                                         double* __restrict__ d, double* __restrict__ e,
                                                         const double2* __restrict__ p_in, double2* __restrict__ p_out) {
-  __shared__ HetrdShared sh;
+  HETRD_SHARED(n);
   const int len = n - j - 1;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // the owned row of this wave (if any) and the old pivot row, straight into registers
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(eig::NT) void k_backtransform(const double2* __rest
                                                            const double2* __restrict__ taus,
                                                            const double* __restrict__ S, int n,
                                                            double2* __restrict__ Zt) {
-  __shared__ double2 z[eig::MAXN];
+  double2* z = hetrd_dyn;  // n entries
   __shared__ double red[2 * (eig::NT / 64) + 2];
   const int k = blockIdx.x;
   for (int i = threadIdx.x; i < n; i += eig::NT) z[i] = make_double2(S[(long long)k * n + i], 0.0);
@@ -321,7 +325,8 @@ __device__ __forceinline__ double recip(double q) {
 __global__ __launch_bounds__(eig::NT) void k_tri_bisect(const double* __restrict__ d, const double* __restrict__ e,
                                                         int n, int m, double* __restrict__ wb,
                                                         double* __restrict__ scal) {
-  __shared__ double dl[eig::MAXN], e2[eig::MAXN];
+  double* dl = tri_dyn;  // n entries each
+  double* e2 = tri_dyn + n;
   __shared__ double red[3 * (eig::NT / 64)];
   __shared__ int first[eig::NT / 64];
   const int k = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -407,11 +412,11 @@ __device__ __forceinline__ double start_value(unsigned long long key) {
 // i+2 (the reciprocal of the pivot, floored at +-tiny), multiplier l[i], and sw[i] != 0
 // when rows i and i+1 were swapped (partial pivoting, as LAPACK's dlagtf).
 struct TriLU {
-  double rinv[eig::MAXN], u1[eig::MAXN], u2[eig::MAXN], l[eig::MAXN], sw[eig::MAXN];
+  double *rinv, *u1, *u2, *l, *sw;
 };
 
 __device__ void tri_factor(const double* __restrict__ dl, const double* __restrict__ el, int n, double lam,
-                           double tiny, TriLU& f) {
+                           double tiny, const TriLU& f) {
   double a = dl[0] - lam, b = n > 1 ? el[0] : 0.0;
   for (int i = 0; i + 1 < n; ++i) {
     const double s = el[i], dd = dl[i + 1] - lam, g = i + 2 < n ? el[i + 1] : 0.0;
@@ -473,9 +478,11 @@ __global__ __launch_bounds__(64) void k_tri_inverse(const double* __restrict__ d
                                                     int n, int m, int round, const double* __restrict__ wb,
                                                     const double* __restrict__ scal, double* __restrict__ S,
                                                     double* __restrict__ work) {
-  __shared__ TriLU f;
-  __shared__ double x[eig::MAXN];
-  __shared__ double dl[eig::MAXN], el[eig::MAXN];
+  // dynamic LDS of 8 n doubles: the factors, the vector and T
+  const TriLU f{tri_dyn, tri_dyn + n, tri_dyn + 2 * n, tri_dyn + 3 * n, tri_dyn + 4 * n};
+  double* x = tri_dyn + 5 * n;
+  double* dl = tri_dyn + 6 * n;
+  double* el = tri_dyn + 7 * n;
   const int r = blockIdx.x, tid = threadIdx.x;
   if (r >= m) return;
   const double tnorm = scal[0];
@@ -721,18 +728,20 @@ static std::vector<TridiagGraph> g_graphs;
 static int hetrd_groups(int n) {
   int g = (n - 2 + 3) / 4;
   if (g < 1) g = 1;
-  if (g > 256) g = 256;
+  if (g > 512) g = 512;
   return g;
 }
 
 static void enqueue_tridiag(hipStream_t s, int n, double2* A, double2* V, double2* taus, double* d, double* e,
                             double2* p0, double2* p1) {
   const int G = hetrd_groups(n);
-  hipLaunchKernelGGL(k_hetrd_start, dim3(G), dim3(eig::NT), 0, s, A, n, V, taus, d, e, p0);
+  hipLaunchKernelGGL(k_hetrd_start, dim3(G), dim3(eig::NT), 3 * (size_t)n * sizeof(double2), s, A, n, V, taus,
+                     d, e, p0);
   for (int j = 0; j <= n - 2; ++j) {
     const double2* pin = (j & 1) ? p1 : p0;
     double2* pout = (j & 1) ? p0 : p1;
-    hipLaunchKernelGGL(k_hetrd_step, dim3(G), dim3(eig::NT), 0, s, A, n, j, V, taus, d, e, pin, pout);
+    hipLaunchKernelGGL(k_hetrd_step, dim3(G), dim3(eig::NT), 3 * (size_t)n * sizeof(double2), s, A, n, j, V,
+                       taus, d, e, pin, pout);
   }
 }
 
@@ -776,7 +785,7 @@ size_t heev_workspace_bytes(int n) {
 int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void* ws, size_t ws_bytes,
                        hipStream_t s) {
   if (n < 2 || n > eig::MAXN || nev < 1 || nev > n) {
-    set_error("heev: need 2 <= n <= 1024 and 1 <= nev <= n");
+    set_error("heev: need 2 <= n <= 2048 and 1 <= nev <= n");
     return 1;
   }
   if (ws_bytes < heev_workspace_bytes(n)) {
@@ -814,9 +823,11 @@ int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void*
     return 1;
   }
   const int m = nev + eig::kClusterMax < n ? nev + eig::kClusterMax : n;
-  hipLaunchKernelGGL(k_tri_bisect, dim3(m), dim3(eig::NT), 0, s, d, e, n, m, wb, scal);
+  hipLaunchKernelGGL(k_tri_bisect, dim3(m), dim3(eig::NT), 2 * (size_t)n * sizeof(double), s, d, e, n, m, wb,
+                     scal);
   for (int round = 0; round < eig::kInverseRounds; ++round) {
-    hipLaunchKernelGGL(k_tri_inverse, dim3(m), dim3(64), 0, s, d, e, n, m, round, wb, scal, S, lu);
+    hipLaunchKernelGGL(k_tri_inverse, dim3(m), dim3(64), 8 * (size_t)n * sizeof(double), s, d, e, n, m, round,
+                       wb, scal, S, lu);
     hipLaunchKernelGGL(k_tri_cluster, dim3(nev), dim3(eig::NT), 0, s, d, e, n, m, nev, round,
                        round + 1 == eig::kInverseRounds ? 1 : 0, wb, scal, w, S, lu);
   }
@@ -824,7 +835,7 @@ int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void*
     set_error("heev: tridiagonal eigenpair launch failed");
     return 1;
   }
-  hipLaunchKernelGGL(k_backtransform, dim3(nev), dim3(eig::NT), 0, s, V, taus, S, n,
+  hipLaunchKernelGGL(k_backtransform, dim3(nev), dim3(eig::NT), (size_t)n * sizeof(double2), s, V, taus, S, n,
                      static_cast<double2*>(zt));
   if (hipGetLastError() != hipSuccess) {
     set_error("heev: back-transformation launch failed");

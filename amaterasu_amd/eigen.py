@@ -294,14 +294,14 @@ def davidson(apply_hamiltonian, guess, kinetic, *, tolerance: float = 1e-8, max_
 #: Projected matrices at least this large go to the library's own Hermitian eigensolver
 #: (device tridiagonalisation); smaller ones to torch.linalg.eigh.
 HEEV_MIN = 96
-HEEV_MAX = 1024
+HEEV_MAX = 2048
 
 
-def _lowest_eigenpairs(proj: torch.Tensor, nbands: int):
+def _lowest_eigenpairs(proj: torch.Tensor, nbands: int, workspace_tag: str = "heev"):
     """(w[:nbands], rows of eigenvectors) of a Hermitian FP64 matrix."""
     m = proj.shape[0]
     if HEEV_MIN <= m <= HEEV_MAX:
-        return ops.heev_lowest(proj.contiguous(), nbands)
+        return ops.heev_lowest(proj.contiguous(), nbands, workspace_tag=workspace_tag)
     w, vec = torch.linalg.eigh(proj)
     return w[:nbands], vec[:, :nbands].T.contiguous()
 

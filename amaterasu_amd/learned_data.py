@@ -30,9 +30,8 @@ import torch
 from .learned import DensityModel, grid_features, learned_density, train
 from .scf import HARTREE_IN_EV, SCFSettings, run_scf
 from .system import attach_ionic_fields, build_system, load_dataset
-from .workloads import si_conventional
+from .workloads import KMESH_SHIFTED_222, si_conventional
 
-KMESH_SHIFTED_222 = np.array([[0.25, 0.25, 0.25], [0.25, 0.25, -0.25], [0.25, -0.25, 0.25], [-0.25, 0.25, 0.25]])
 ECUT = 15.0
 TOL = SCFSettings(tol_energy=1e-6 * HARTREE_IN_EV, tol_density=1e-6, max_steps=60)
 

@@ -162,9 +162,10 @@ def precondition(res: torch.Tensor, v: torch.Tensor, kinetic: torch.Tensor, rows
     return out
 
 
-def heev_lowest(a: torch.Tensor, nev: int):
+def heev_lowest(a: torch.Tensor, nev: int, *, workspace_tag: str = "heev"):
     """Lowest ``nev`` eigenpairs of a Hermitian complex128 matrix on the device: (w, zt) with
-    w ascending (float64) and zt[k] the k-th eigenvector (row). See amt_heev_lowest."""
+    w ascending (float64) and zt[k] the k-th eigenvector (row). See amt_heev_lowest. Calls
+    that may run at the same time (on different streams) need different ``workspace_tag``s."""
     _require(a, "a")
     if a.dtype != torch.complex128 or a.dim() != 2 or a.shape[0] != a.shape[1]:
         raise AmtError("heev: a must be a square complex128 device matrix")
@@ -174,7 +175,7 @@ def heev_lowest(a: torch.Tensor, nev: int):
     w = torch.empty(nev, dtype=torch.float64, device=a.device)
     zt = torch.empty((nev, n), dtype=torch.complex128, device=a.device)
     nbytes = int(lib().amt_heev_workspace(n))
-    ws = workspace(nbytes, "heev")
+    ws = workspace(nbytes, workspace_tag)
     check(lib().amt_heev_lowest(n, a.data_ptr(), nev, w.data_ptr(), zt.data_ptr(), ws.data_ptr(), nbytes,
                                 stream_handle()), "heev")
     return w, zt

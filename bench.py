@@ -205,6 +205,61 @@ def si64_scf(log=None) -> dict:
     return res
 
 
+def si214_scf(log=None) -> dict:
+    """The largest combined case (opt-in, about ten minutes): Si214 (3x3x3 conventional with two
+    seeded vacancies, sigma 0.05 A), PBE, ecut 15 Ha, grid 108^3, ~81k plane waves, the four
+    k-points of the time-reversal-reduced shifted 2x2x2 mesh batched in one Davidson, 440 bands,
+    Fermi-Dirac smearing 0.01 Ry, SCF to 1e-6 Ha/atom and density residual 1e-6: plain FP64
+    from the atomic start against FP32 H.psi/GEMMs (FP64 Cholesky, eigensolve, mixing) from
+    the learned start. There is no warm-up (one SCF iteration here is a minute of work): each
+    form pays its own first-use costs (GEMM split tuning) inside its timed run. The FP64 planes
+    of 108^3 do not fit one workgroup's LDS, so the FP64 form's local H.psi and density take
+    the full-grid transforms."""
+    import dataclasses
+
+    import numpy as np
+    import torch
+
+    from amaterasu_amd.forces import compute_forces
+    from amaterasu_amd.learned import MODEL_PATH, DensityModel, learned_density
+    from amaterasu_amd.scf import HARTREE_IN_EV, SCFSettings, run_scf
+    from amaterasu_amd.workloads import si214_kgrid_system
+
+    t0 = time.perf_counter()
+    system = si214_kgrid_system()
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    base = SCFSettings(tol_energy=1e-6 * HARTREE_IN_EV, tol_density=1e-6, max_steps=60, scheme="fermi-dirac",
+                       smearing=0.005)
+    learned = MODEL_PATH.exists()
+    model = DensityModel.load() if learned else None
+    runs = {}
+    for name, prec, use_learned in (("fp64", "fp64", False), ("mixed_learned", "fp32", True)):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rho = learned_density(system, model) if (use_learned and learned) else system.initial_density()
+        r = run_scf(system, dataclasses.replace(base, precision=prec), density=rho,
+                    log=(lambda line, name=name: log(f"si214 {name}: {line}")) if log else None)
+        torch.cuda.synchronize()
+        runs[name] = (time.perf_counter() - t, r, compute_forces(system, r.density, r.wavefunctions, r.occupations))
+        if log:
+            log(f"si214 scf {name}: {runs[name][0]:.2f} s, {r.iterations} iterations")
+    (ta, a, fa), (tb, b, fb) = runs["fp64"], runs["mixed_learned"]
+    occupied = a.occupations > 0.5
+    ev_per_ang = HARTREE_IN_EV / 0.5291772105638411
+    res = {"workload": "Si214 (3x3x3 conventional, 2 seeded vacancies, sigma 0.05 A), PBE, ecut 15 Ha, 108^3, "
+                       "shifted 2x2x2 mesh (4 k, batched), 440 bands, Fermi-Dirac 0.01 Ry, tol 1e-6 Ha/atom + density "
+                       "residual 1e-6",
+           "setup_seconds": round(setup, 1), "learned_start": learned, "speedup": round(ta / tb, 3),
+           "free_energy_difference_ha_per_atom": abs(a.energies.free_energy - b.energies.free_energy) / len(system.symbols),
+           "occupied_eigenvalue_max_difference_ha": float(np.abs(a.eigenvalues - b.eigenvalues)[occupied].max()),
+           "force_max_difference_ev_per_ang": float(np.abs(fa - fb).max() * ev_per_ang)}
+    for name, (t, r, _) in runs.items():
+        res[name] = {"seconds": round(t, 2), "iterations": r.iterations, "davidson_steps": r.davidson_steps,
+                     "converged": r.converged, "free_energy_ha": r.energies.free_energy}
+    return res
+
+
 def pmc_probe() -> None:
     """Three launches of the plane kernel on the k-batch workload and one elementwise kernel
     that reads and writes 256 MiB, for the FETCH_SIZE / WRITE_SIZE counter runs (the
@@ -278,6 +333,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-scf", action="store_true", help="skip the Si64 SCF wall-time comparison")
     ap.add_argument("--pmc-probe", action="store_true", help="only the short program for counter runs")
+    ap.add_argument("--si214", action="store_true", help="add the Si214 k-grid SCF comparison (about ten minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the k-point/band sharding across "
@@ -351,6 +407,8 @@ def main() -> int:
     }
     if not args.no_scf:
         line["si64_scf"] = si64_scf(log=lambda m: print(m, file=sys.stderr, flush=True))
+    if args.si214:
+        line["si214_scf"] = si214_scf(log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec, v, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -132,7 +132,7 @@ int amt_kb_dij(int precision, const void* proj, const void* dij, const int* blk_
                void* out, int nrow, int nproj, long long sp, long long sd, long long so, int batch, void* stream);
 
 /*
- * Lowest nev eigenpairs of a dense Hermitian n x n complex128 matrix (row-major, 2 <= n <= 1024):
+ * Lowest nev eigenpairs of a dense Hermitian n x n complex128 matrix (row-major, 2 <= n <= 2048):
  * the subspace eigensolve of rayleigh_ritz (scf/eigen/subspace.py:74, scipy.linalg.eigh there).
  * w: nev eigenvalues ascending (float64, device); zt: nev x n, row k = eigenvector k (device).
  * Householder tridiagonalisation on the device (one launch per column); the lowest

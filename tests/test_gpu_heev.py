@@ -21,7 +21,8 @@ def _hermitian(n, seed, degenerate=False):
     return (x + x.conj().T) / 2
 
 
-@pytest.mark.parametrize("n,nev", [(2, 1), (3, 3), (17, 5), (160, 40), (333, 100), (640, 160), (1024, 256)])
+@pytest.mark.parametrize("n,nev", [(2, 1), (3, 3), (17, 5), (160, 40), (333, 100), (640, 160), (1024, 256),
+                                   (1760, 440)])
 def test_heev_matches_lapack(n, nev):
     from amaterasu_amd import ops
 
@@ -55,7 +56,7 @@ def test_heev_leaves_input_and_rejects_bad_sizes():
     ops.heev_lowest(a, 10)
     assert torch.equal(a, keep)
     with pytest.raises(AmtError):
-        ops.heev_lowest(torch.zeros((1100, 1100), dtype=torch.complex128, device="cuda"), 4)
+        ops.heev_lowest(torch.zeros((2100, 2100), dtype=torch.complex128, device="cuda"), 4)
     with pytest.raises(AmtError):
         ops.heev_lowest(a, 51)
 
