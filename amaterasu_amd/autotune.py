@@ -2,15 +2,18 @@
 
 ``amt_zgemm`` splits a long-k product over several workgroups per output tile when the
 output alone has too few tiles to fill the GPU, with a deterministic second-pass
-reduction; how many splits pay depends on the shape (tiles, k) and on the machine. The
-first time a product of at least ``MIN_GFLOP`` falls in a new bucket -- (precision, op,
-ceil(m/64), ceil(n/64), ceil(k/256), batch), the key the library uses -- this module times
-the candidate splits on the caller's stream with HIP events (one warm-up and ``REPEATS``
-timed runs each, into a scratch output so the caller's data is untouched), records the
-fastest with ``amt_zgemm_set_split`` and remembers the bucket for the rest of the process.
-The result of a product does not depend on the split beyond rounding.
+reduction; how many splits pay depends on the shape (tiles, k) and on the machine. With
+``AMT_AUTOTUNE=1``, the first time a product of at least ``MIN_GFLOP`` falls in a new bucket --
+(precision, op, ceil(m/64), ceil(n/64), ceil(k/256), batch), the key the library uses -- this
+module times the built-in heuristic's split and its neighbours within a factor of four on the
+caller's stream with HIP events (one warm-up and ``REPEATS`` timed runs each, into a scratch
+output so the caller's data is untouched), records the fastest with ``amt_zgemm_set_split``
+and remembers the bucket for the rest of the process. The result of a product does not
+depend on the split beyond rounding.
 
-``AMT_AUTOTUNE=0`` turns it off (the built-in heuristic is used throughout).
+It is off by default: on the Si64 and Si8 shapes the heuristic was within a few per cent of
+the best candidate, and tuning the many buckets of a short run (band counts change as bands
+converge) costs more than it saves unless the process runs long enough to reuse them.
 """
 
 from __future__ import annotations
@@ -23,13 +26,22 @@ from ._lib import check, lib
 
 MIN_GFLOP = 2.0
 REPEATS = 3
-CANDIDATES = (1, 2, 4, 8, 16, 32, 64, 128)
+NEIGHBOURS = (0.25, 0.5, 1.0, 2.0, 4.0)  # multiples of the heuristic split that are timed
 
 _TUNED: dict = {}
 
 
 def enabled() -> bool:
-    return os.environ.get("AMT_AUTOTUNE", "1") != "0"
+    return os.environ.get("AMT_AUTOTUNE", "0") == "1"
+
+
+def heuristic_split(m: int, n: int, k: int, batch: int) -> int:
+    """The library's built-in choice (``choose_split`` in gemm.hip): about 1024 workgroups of
+    64 x 64 tiles, at least 256 of k per split, at most 256 splits."""
+    tiles = ((m + 63) // 64) * ((n + 63) // 64) * batch
+    if tiles >= 512:
+        return 1
+    return int(max(1, min((1024 + tiles - 1) // tiles, (k + 255) // 256, 256)))
 
 
 def bucket(prec: int, op: int, m: int, n: int, k: int, batch: int) -> tuple:
@@ -51,9 +63,14 @@ def tune_zgemm(run, prec: int, op: int, m: int, n: int, k: int, batch: int) -> i
         _TUNED[key] = 0
         return 0
     kmax = max(1, (k + 255) // 256)
+    h = heuristic_split(m, n, k, batch)
+    if h == 1 and kmax == 1:
+        _TUNED[key] = 0
+        return 0
     best, best_ms = 0, float("inf")
     stream = torch.cuda.current_stream()
-    for split in sorted({c for c in CANDIDATES if c <= kmax} | {0}):
+    candidates = {min(kmax, 256, max(1, int(round(h * f)))) for f in NEIGHBOURS} - {h}
+    for split in [0] + sorted(candidates):
         check(lib().amt_zgemm_set_split(prec, op, m, n, k, batch, split), "zgemm_set_split")
         run()  # warm-up (and workspace growth)
         start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -70,4 +87,4 @@ def tune_zgemm(run, prec: int, op: int, m: int, n: int, k: int, batch: int) -> i
     return best
 
 
-__all__ = ["CANDIDATES", "MIN_GFLOP", "bucket", "enabled", "tune_zgemm", "tuned_splits"]
+__all__ = ["MIN_GFLOP", "NEIGHBOURS", "bucket", "enabled", "heuristic_split", "tune_zgemm", "tuned_splits"]

@@ -26,9 +26,6 @@
 // 3. Back-transformation Z = Q S for the lowest nev eigenvectors: one workgroup per
 //    eigenvector applies the reflectors in reverse order (no global synchronisation).
 #include <cfloat>
-#include <cstdlib>
-#include <mutex>
-#include <vector>
 
 #include "kernels.hpp"
 
@@ -155,8 +152,10 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_start(const double2* __restri
 // the wave keeps that row in registers from its update to its product with v_{j+1}, and the
 // old pivot row j+1 is fetched into registers at the start, so the only global round trips
 // on the critical path are the loads of v_j, p_j and the owned row.
-constexpr int kRowRegs = eig::MAXN / 64;  // elements of one row per lane
+// kRowRegs = elements of one row per lane: 16 up to n = 1024, 32 up to n = 2048 (the larger
+// register file costs ~20 % per launch, so it is only used where it is needed).
 
+template <int kRowRegs>
 __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A, int n, int j,
                                                         double2* __restrict__ V, double2* __restrict__ taus,
                                                         double* __restrict__ d, double* __restrict__ e,
@@ -712,18 +711,6 @@ __global__ __launch_bounds__(eig::NT) void k_tri_cluster(const double* __restric
 }
 
 // ------------------------------------------------------------------ host
-// The n launches of the tridiagonalisation, captured once per (device, workspace, n) into
-// a HIP graph and replayed: the launch sequence is fixed by n and every pointer lives in the
-// caller's workspace, so only the input copy (outside the graph) differs between calls.
-struct TridiagGraph {
-  int dev;
-  const void* ws;
-  int n;
-  hipGraphExec_t exec;
-};
-static std::mutex g_graph_mutex;
-static std::vector<TridiagGraph> g_graphs;
-
 // Workgroups of the tridiagonalisation: one trailing row per wave at most (n - 2 <= 4 G).
 static int hetrd_groups(int n) {
   int g = (n - 2 + 3) / 4;
@@ -740,40 +727,13 @@ static void enqueue_tridiag(hipStream_t s, int n, double2* A, double2* V, double
   for (int j = 0; j <= n - 2; ++j) {
     const double2* pin = (j & 1) ? p1 : p0;
     double2* pout = (j & 1) ? p0 : p1;
-    hipLaunchKernelGGL(k_hetrd_step, dim3(G), dim3(eig::NT), 3 * (size_t)n * sizeof(double2), s, A, n, j, V,
-                       taus, d, e, pin, pout);
+    if (n <= eig::MAXN / 2)
+      hipLaunchKernelGGL(k_hetrd_step<eig::MAXN / 128>, dim3(G), dim3(eig::NT), 3 * (size_t)n * sizeof(double2), s,
+                         A, n, j, V, taus, d, e, pin, pout);
+    else
+      hipLaunchKernelGGL(k_hetrd_step<eig::MAXN / 64>, dim3(G), dim3(eig::NT), 3 * (size_t)n * sizeof(double2), s,
+                         A, n, j, V, taus, d, e, pin, pout);
   }
-}
-
-static hipGraphExec_t tridiag_graph(int n, const void* ws, double2* A, double2* V, double2* taus, double* d,
-                                    double* e, double2* p0, double2* p1) {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  std::lock_guard<std::mutex> lock(g_graph_mutex);
-  for (const TridiagGraph& g : g_graphs)
-    if (g.dev == dev && g.ws == ws && g.n == n) return g.exec;
-  hipStream_t cs = nullptr;
-  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
-  if (ok) {
-    enqueue_tridiag(cs, n, A, V, taus, d, e, p0, p1);
-    ok = hipStreamEndCapture(cs, &graph) == hipSuccess && graph != nullptr;
-  }
-  if (ok) ok = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
-  if (graph) (void)hipGraphDestroy(graph);
-  (void)hipStreamDestroy(cs);
-  if (!ok) {
-    (void)hipGetLastError();
-    return nullptr;
-  }
-  if (g_graphs.size() >= 32) {  // bounded cache: drop the oldest
-    (void)hipGraphExecDestroy(g_graphs.front().exec);
-    g_graphs.erase(g_graphs.begin());
-  }
-  g_graphs.push_back({dev, ws, n, exec});
-  return exec;
 }
 
 size_t heev_workspace_bytes(int n) {
@@ -808,16 +768,7 @@ int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void*
     set_error("heev: copy of the input failed");
     return 1;
   }
-  const char* nog = std::getenv("AMT_HEEV_NO_GRAPH");
-  hipGraphExec_t exec = (nog && nog[0] == '1') ? nullptr : tridiag_graph(n, ws, A, V, taus, d, e, p0, p1);
-  if (exec) {
-    if (hipGraphLaunch(exec, s) != hipSuccess) {
-      set_error("heev: tridiagonalisation graph launch failed");
-      return 1;
-    }
-  } else {
-    enqueue_tridiag(s, n, A, V, taus, d, e, p0, p1);
-  }
+  enqueue_tridiag(s, n, A, V, taus, d, e, p0, p1);
   if (hipGetLastError() != hipSuccess) {
     set_error("heev: tridiagonalisation launch failed");
     return 1;

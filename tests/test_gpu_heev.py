@@ -62,8 +62,8 @@ def test_heev_leaves_input_and_rejects_bad_sizes():
 
 
 def test_heev_repeated_calls_replay_correctly():
-    """The tridiagonalisation is replayed from a cached graph: consecutive calls of one size
-    on different matrices (with other sizes in between) must each match LAPACK."""
+    """Consecutive calls of one size on different matrices, with other sizes in between and
+    the workspace reused, must each match LAPACK."""
     from amaterasu_amd import ops
 
     for seed in (11, 12, 13):
