@@ -18,6 +18,14 @@
 // LDS, stored k-major so fragment reads are unit-stride across lanes. Long-K /
 // small-output products (projections onto a few hundred bands) are split along K
 // into enough workgroups to fill 256 CUs, with a deterministic second-pass reduction.
+//
+// FP32 accumulation over a long K chunk is two-level: the MFMA's FP32 accumulators cover
+// kFlushTiles * BK of K and are then added into FP64 registers, so the rounding error of a
+// long contraction (subspace overlaps over ~10^5 plane waves) grows with that length, not
+// with K. One FP32 accumulator over all of K = 81k left relative errors of ~6e-5 in the
+// Gram matrices of a 440-band Davidson, enough to stall an SCF. Chunks of at most
+// kLongChunk (every split-K launch of the projector products) keep the single level and
+// its higher occupancy; split partials are summed in FP64 either way.
 #include <array>
 #include <map>
 #include <mutex>
@@ -50,8 +58,10 @@ struct Mfma<double> {
 };
 
 constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int kFlushTiles = 4;
+constexpr int kLongChunk = 1024;
 
-template <typename R, int OP>
+template <typename R, int OP, bool kTwoLevel>
 __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, int nsplit,
                                                const typename Cplx<R>::T* __restrict__ A, long long lda,
                                                long long sA, const typename Cplx<R>::T* __restrict__ B,
@@ -81,6 +91,31 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
         accr[a][b][q] = 0;
         acci[a][b][q] = 0;
       }
+  // second-level FP64 sums of the FP32 accumulators
+  double sumr[2][2][4], sumi[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        sumr[a][b][q] = 0.0;
+        sumi[a][b][q] = 0.0;
+      }
+  auto flush = [&]() {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          sumr[a][b][q] += (double)accr[a][b][q];
+          sumi[a][b][q] += (double)acci[a][b][q];
+          accr[a][b][q] = 0;
+          acci[a][b][q] = 0;
+        }
+  };
+  int tiles = 0;
   const C zero = cmake<C>(0, 0);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
     {
@@ -131,7 +166,14 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
         }
     }
     __syncthreads();
+    if constexpr (kTwoLevel) {
+      if (++tiles == kFlushTiles) {
+        flush();
+        tiles = 0;
+      }
+    }
   }
+  if constexpr (kTwoLevel) flush();
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -141,7 +183,8 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
         const int row = m0 + wr + rb * 16 + Mfma<R>::row(lane, q);
         const int col = n0 + wc + cb * 16 + (lane & 15);
         if (row < M && col < N) {
-          const C v = cmake<C>(accr[rb][cb][q], acci[rb][cb][q]);
+          const C v = kTwoLevel ? cmake<C>((R)sumr[rb][cb][q], (R)sumi[rb][cb][q])
+                                : cmake<C>(accr[rb][cb][q], acci[rb][cb][q]);
           if (part) {
             part[(((long long)batch * nsplit + split) * M + row) * N + col] = v;
           } else {
@@ -167,7 +210,7 @@ __global__ void k_splitk_reduce(int M, int N, int nsplit, const typename Cplx<R>
   const int batch = blockIdx.y;
   if (e >= (long long)M * N) return;
   const int row = (int)(e / N), col = (int)(e % N);
-  R sr = 0, si = 0;
+  double sr = 0, si = 0;
   const C* p = part + (long long)batch * nsplit * M * N + e;
   for (int s = 0; s < nsplit; ++s) {
     const C v = p[(long long)s * M * N];
@@ -175,7 +218,7 @@ __global__ void k_splitk_reduce(int M, int N, int nsplit, const typename Cplx<R>
     si += v.y;
   }
   C* dst = Cm + (long long)batch * sC + (long long)row * ldc + col;
-  C o = cmake<C>(alpha * sr, alpha * si);
+  C o = cmake<C>((R)(alpha * sr), (R)(alpha * si));
   if (beta != (R)0) {
     const C old = *dst;
     o.x += beta * old.x;
@@ -264,12 +307,20 @@ static int zgemm_t(int op, int m, int n, int k, double alpha, const void* a, lon
     return 1;
   }
   const dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM, batch * ns);
-  if (op == 0)
-    hipLaunchKernelGGL((k_zgemm<R, 0>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
-                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, ns > 1 ? part : nullptr);
+  const bool two = std::is_same<R, float>::value && kchunk > kLongChunk;
+  C* const p = ns > 1 ? part : nullptr;
+  if (op == 0 && two)
+    hipLaunchKernelGGL((k_zgemm<R, 0, true>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
+                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, p);
+  else if (op == 0)
+    hipLaunchKernelGGL((k_zgemm<R, 0, false>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
+                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, p);
+  else if (two)
+    hipLaunchKernelGGL((k_zgemm<R, 1, true>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
+                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, p);
   else
-    hipLaunchKernelGGL((k_zgemm<R, 1>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
-                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, ns > 1 ? part : nullptr);
+    hipLaunchKernelGGL((k_zgemm<R, 1, false>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
+                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, p);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error(std::string("zgemm launch: ") + hipGetErrorString(e));

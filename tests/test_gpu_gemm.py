@@ -79,3 +79,31 @@ def test_zgemm_strided_views():
     ops.zgemm(rot, view, 0, out=out[:, 2:7])
     assert torch.allclose(out[:, 2:7].cpu(), rot.cpu() @ view.cpu(), rtol=1e-12, atol=1e-11)
     assert float(out[:, :2].abs().max()) == 0.0 and float(out[:, 7:].abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("split", [1, 0])
+def test_zgemm_fp32_long_contraction_accumulates_in_fp64(split):
+    """Gram matrix of unit rows over K = 131072 plane waves in FP32 against the FP64 product
+    of the same (FP32-rounded) inputs. With one split (the whole contraction in one
+    workgroup) the FP32 matrix-core sums are carried into FP64 every 64 of K; with the
+    heuristic split (256 chunks of 512) each chunk keeps one FP32 accumulator and the
+    partials are summed in FP64. Either way the error stays near the final rounding
+    (~1e-7), where one FP32 accumulator over all of K loses ~1e-4 on the unit diagonal."""
+    import torch
+
+    from amaterasu_amd import ops
+    from amaterasu_amd._lib import FP32, check, lib
+
+    m, k = 128, 131072
+    g = torch.Generator().manual_seed(9)
+    a = torch.randn((m, k), dtype=torch.complex128, generator=g)
+    a = (a / torch.linalg.vector_norm(a, dim=1, keepdim=True)).to(torch.complex64).cuda()
+    a64 = a.to(torch.complex128)
+    check(lib().amt_zgemm_set_split(FP32, 1, m, m, k, 1, split), "zgemm_set_split")
+    try:
+        got = ops.zgemm(a, a, 1).to(torch.complex128)
+    finally:
+        check(lib().amt_zgemm_set_split(FP32, 1, m, m, k, 1, 0), "zgemm_set_split")
+    want = ops.zgemm(a64, a64, 1)
+    assert float((torch.diagonal(want) - 1.0).abs().max()) < 1e-6
+    assert float((got - want).abs().max()) < 3e-7
