@@ -146,24 +146,52 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_start(const double2* __restri
   }
 }
 
+// Block-wide sum of a double2 with one barrier: every thread adds the per-wave partials
+// itself, in the same order. red (2 * NT/64 doubles) must not be reused in the same launch.
+__device__ __forceinline__ double2 block_sum2_once(double2 v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_down(v.x, off, 64);
+    v.y += __shfl_down(v.y, off, 64);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[2 * wave] = v.x;
+    red[2 * wave + 1] = v.y;
+  }
+  __syncthreads();
+  double sx = 0.0, sy = 0.0;
+#pragma unroll
+  for (int w = 0; w < eig::NT / 64; ++w) {
+    sx += red[2 * w];
+    sy += red[2 * w + 1];
+  }
+  return make_double2(sx, sy);
+}
+
 // Launch j (0 <= j <= n-2): finish step j, start step j+1. Trailing numbering of step j:
 // index t <-> row/column j+1+t. p_in: p_j (length n-j-1); p_out: p_{j+1}.
 // Each workgroup owns at most one trailing row per wave (n - 2 <= 4 G, checked by the host):
-// the wave keeps that row in registers from its update to its product with v_{j+1}, and the
-// old pivot row j+1 is fetched into registers at the start, so the only global round trips
-// on the critical path are the loads of v_j, p_j and the owned row.
-// kRowRegs = elements of one row per lane: 16 up to n = 1024, 32 up to n = 2048 (the larger
-// register file costs ~20 % per launch, so it is only used where it is needed).
+// the wave keeps that row in registers from its update to its product with v_{j+1}. Thread
+// x holds elements t = x + NT q of v_j, p_j, w_j and of the pivot row j+1 in registers, so
+// all global loads are issued at once at the start, and the launch has three barriers: the
+// sum p^H v, the publication of w, and the norm of the new column. The reflector of column
+// j+1 (zlarfg) is kept unscaled in LDS; its scale enters the product with the owned row
+// once, after the row's dot product, and the stored V row.
+// kRowRegs = elements of one row per lane: 16 up to n = 1024, 32 up to n = 2048.
 
 template <int kRowRegs>
 __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A, int n, int j,
                                                         double2* __restrict__ V, double2* __restrict__ taus,
                                                         double* __restrict__ d, double* __restrict__ e,
                                                         const double2* __restrict__ p_in, double2* __restrict__ p_out) {
-  HETRD_SHARED(n);
+  constexpr int QV = kRowRegs / 4;  // vector elements per thread (n <= NT * QV)
+  double2* sv = hetrd_dyn;          // v_j
+  double2* sw = hetrd_dyn + n;      // w_j
+  double2* sx = hetrd_dyn + 2 * n;  // column j+1 below the diagonal, unscaled
+  __shared__ double red[2][2 * (eig::NT / 64)];
   const int len = n - j - 1;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // the owned row of this wave (if any) and the old pivot row, straight into registers
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const double2 zero = make_double2(0.0, 0.0);
   const int myrow = j + 2 + blockIdx.x + wave * gridDim.x;
   const bool own = myrow < n;
   double2 rowv[kRowRegs];
@@ -172,97 +200,125 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A,
     const int c = j + 1 + lane + 64 * q;
     rowv[q] = (own && c < n) ? A[(long long)myrow * n + c] : make_double2(0.0, 0.0);
   }
-  double2 piv[kRowRegs / 4];  // pivot row j+1, columns j+1 + threadIdx.x + 256 q
+  double2 piv[QV], vq[QV], pq[QV];
 #pragma unroll
-  for (int q = 0; q < kRowRegs / 4; ++q) {
-    const int c = j + 1 + threadIdx.x + eig::NT * q;
-    piv[q] = c < n ? A[(long long)(j + 1) * n + c] : make_double2(0.0, 0.0);
+  for (int q = 0; q < QV; ++q) {
+    const int t = tid + eig::NT * q;
+    piv[q] = t < len ? A[(long long)(j + 1) * n + j + 1 + t] : make_double2(0.0, 0.0);
+    vq[q] = t < len ? V[(long long)j * n + t] : make_double2(0.0, 0.0);
+    pq[q] = t < len ? p_in[t] : make_double2(0.0, 0.0);
   }
   const double2 tau = taus[j];
-  for (int t = threadIdx.x; t < len; t += eig::NT) {
-    sh.v[t] = V[(long long)j * n + t];
-    sh.w[t] = p_in[t];
-  }
-  __syncthreads();
   // K = -(tau/2) p^H v; w = p + K v
-  double2 part = make_double2(0.0, 0.0);
-  for (int t = threadIdx.x; t < len; t += eig::NT) {
-    const double2 pv = zmul(zconj(sh.w[t]), sh.v[t]);
+  double2 part = zero;
+#pragma unroll
+  for (int q = 0; q < QV; ++q) {
+    const int t = tid + eig::NT * q;
+    if (t < len) sv[t] = vq[q];
+    const double2 pv = zmul(zconj(pq[q]), vq[q]);
     part.x += pv.x;
     part.y += pv.y;
   }
-  const double2 phv = block_sum2(part, sh.red);
+  const double2 phv = block_sum2_once(part, red[0]);
   const double2 K = zmul(make_double2(-0.5 * tau.x, -0.5 * tau.y), phv);
-  for (int t = threadIdx.x; t < len; t += eig::NT) {
-    const double2 kv = zmul(K, sh.v[t]);
-    sh.w[t] = make_double2(sh.w[t].x + kv.x, sh.w[t].y + kv.y);
+  double2 wq[QV];
+#pragma unroll
+  for (int q = 0; q < QV; ++q) {
+    const int t = tid + eig::NT * q;
+    const double2 kv = zmul(K, vq[q]);
+    wq[q] = make_double2(pq[q].x + kv.x, pq[q].y + kv.y);
+    if (t < len) sw[t] = wq[q];
   }
   __syncthreads();
   // update the owned row in registers and store it:  A -= v w^H + w v^H
   if (own) {
     const int tr = myrow - j - 1;
-    const double2 vr = sh.v[tr], wr = sh.w[tr];
+    const double2 vr = sv[tr], wr = sw[tr];
     double2* row = A + (long long)myrow * n;
 #pragma unroll
     for (int q = 0; q < kRowRegs; ++q) {
       const int c = j + 1 + lane + 64 * q;
       if (c < n) {
         const int tc = c - j - 1;
-        const double2 a = zmul(vr, zconj(sh.w[tc]));
-        const double2 b = zmul(wr, zconj(sh.v[tc]));
+        const double2 a = zmul(vr, zconj(sw[tc]));
+        const double2 b = zmul(wr, zconj(sv[tc]));
         rowv[q].x -= a.x + b.x;
         rowv[q].y -= a.y + b.y;
         row[c] = rowv[q];
       }
     }
   }
+  const double2 v0 = sv[0], w0 = sw[0];
   const int len2 = n - j - 2;  // length of column j+1 below the diagonal
   if (len2 <= 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      const double2 a = zmul(sh.v[0], zconj(sh.w[0]));
+    if (blockIdx.x == 0 && tid == 0) {
+      const double2 a = zmul(v0, zconj(w0));
       d[j + 1] = piv[0].x - 2.0 * a.x;
     }
     return;
   }
-  // the updated pivot row j+1: old - (v_0 conj(w_c) + w_0 conj(v_c)), v_0 = 1
+  // the updated pivot row j+1 (element t = column j+1+t): old - (v_0 conj(w_t) + w_0 conj(v_t));
+  // the column below the diagonal is its conjugate from t = 1 on: x[t-1]
+  double2 xq[QV];
+  double nrm2 = 0.0;
 #pragma unroll
-  for (int q = 0; q < kRowRegs / 4; ++q) {
-    const int c = j + 1 + threadIdx.x + eig::NT * q;
-    if (c < n) {
-      const int tc = c - j - 1;
-      const double2 a = zmul(sh.v[0], zconj(sh.w[tc]));
-      const double2 b = zmul(sh.w[0], zconj(sh.v[tc]));
-      sh.nr[tc] = make_double2(piv[q].x - a.x - b.x, piv[q].y - a.y - b.y);
+  for (int q = 0; q < QV; ++q) {
+    const int t = tid + eig::NT * q;
+    const double2 a = zmul(v0, zconj(wq[q]));
+    const double2 b = zmul(w0, zconj(vq[q]));
+    const double2 nr = make_double2(piv[q].x - a.x - b.x, piv[q].y - a.y - b.y);
+    xq[q] = zconj(nr);
+    if (t == 0) {
+      if (blockIdx.x == 0) d[j + 1] = nr.x;
+    } else if (t < len) {
+      sx[t - 1] = xq[q];
+      if (t >= 2) nrm2 += nr.x * nr.x + nr.y * nr.y;
     }
   }
-  __syncthreads();
-  if (blockIdx.x == 0 && threadIdx.x == 0) d[j + 1] = sh.nr[0].x;
-  // column j+1 below the diagonal = conj(new row j+1, c >= j+2) -> reflector v_{j+1} (in sh.v)
-  for (int t = threadIdx.x; t < len2; t += eig::NT) sh.v[t] = zconj(sh.nr[1 + t]);
-  __syncthreads();
+  const double xnorm2 = block_sum2_once(make_double2(nrm2, 0.0), red[1]).x;
+  // zlarfg on x: beta, tau2 and the scale 1 / (alpha - beta) of x[1..]
+  const double2 alpha = sx[0];
   double beta;
-  double2 tau2;
-  larfg(sh.v, len2, sh.red, &beta, &tau2);
+  double2 tau2, sc;
+  if (xnorm2 == 0.0 && alpha.y == 0.0) {
+    beta = alpha.x;
+    tau2 = zero;
+    sc = zero;  // x[1..] = 0: H = I
+  } else {
+    const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
+    beta = alpha.x >= 0.0 ? -nrm : nrm;
+    tau2 = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+    const double2 dd = make_double2(alpha.x - beta, alpha.y);
+    const double den = dd.x * dd.x + dd.y * dd.y;
+    sc = make_double2(dd.x / den, -dd.y / den);
+  }
   if (blockIdx.x == 0) {
-    for (int t = threadIdx.x; t < len2; t += eig::NT)
-      V[(long long)(j + 1) * n + t] = t == 0 ? make_double2(1.0, 0.0) : sh.v[t];
-    if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < QV; ++q) {
+      const int t = tid + eig::NT * q;  // x index t - 1
+      if (t >= 1 && t < len) V[(long long)(j + 1) * n + t - 1] = t == 1 ? make_double2(1.0, 0.0) : zmul(xq[q], sc);
+    }
+    if (tid == 0) {
       taus[j + 1] = tau2;
       e[j + 1] = beta;
     }
   }
-  // p_{j+1} for the owned row: tau2 * sum_{c >= j+2} A[r][c] v_{j+1}[c - j - 2], from registers
+  // p_{j+1} for the owned row: tau2 (A[r][j+2] + sc sum_{c >= j+3} A[r][c] x[c - j - 2])
   if (own) {
-    double2 acc = make_double2(0.0, 0.0);
+    double2 a0 = zero, a1 = zero;
 #pragma unroll
     for (int q = 0; q < kRowRegs; ++q) {
       const int c = j + 1 + lane + 64 * q;
-      if (c >= j + 2 && c < n) {
-        const double2 vv = c == j + 2 ? make_double2(1.0, 0.0) : sh.v[c - j - 2];
-        acc.x += rowv[q].x * vv.x - rowv[q].y * vv.y;
-        acc.y += rowv[q].x * vv.y + rowv[q].y * vv.x;
+      if (c == j + 2) {
+        a0 = rowv[q];
+      } else if (c > j + 2 && c < n) {
+        const double2 xv = sx[c - j - 2];
+        a1.x += rowv[q].x * xv.x - rowv[q].y * xv.y;
+        a1.y += rowv[q].x * xv.y + rowv[q].y * xv.x;
       }
     }
+    const double2 s1 = zmul(sc, a1);
+    double2 acc = make_double2(a0.x + s1.x, a0.y + s1.y);
     for (int off = 32; off > 0; off >>= 1) {
       acc.x += __shfl_down(acc.x, off, 64);
       acc.y += __shfl_down(acc.y, off, 64);

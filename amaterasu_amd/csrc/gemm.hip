@@ -116,7 +116,6 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
         }
   };
   int tiles = 0;
-  const C zero = cmake<C>(0, 0);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
     {
       const int row = tid >> 2, kq = (tid & 3) * 4;
@@ -124,7 +123,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int gk = k0 + kq + u;
-        As[kq + u][row] = (gm < M && gk < kend) ? A[(long long)gm * lda + gk] : zero;
+        As[kq + u][row] = (gm < M && gk < kend) ? A[(long long)gm * lda + gk] : cmake<C>(0, 0);
       }
     }
     if (OP == 0) {
@@ -133,7 +132,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int gn = n0 + c0 + u;
-        Bs[kk][c0 + u] = (gk < kend && gn < N) ? B[(long long)gk * ldb + gn] : zero;
+        Bs[kk][c0 + u] = (gk < kend && gn < N) ? B[(long long)gk * ldb + gn] : cmake<C>(0, 0);
       }
     } else {
       const int col = tid >> 2, kq = (tid & 3) * 4;
@@ -141,7 +140,7 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int gk = k0 + kq + u;
-        C v = (gn < N && gk < kend) ? B[(long long)gn * ldb + gk] : zero;
+        C v = (gn < N && gk < kend) ? B[(long long)gn * ldb + gk] : cmake<C>(0, 0);
         v.y = -v.y;
         Bs[kq + u][col] = v;
       }
@@ -279,6 +278,19 @@ size_t zgemm_workspace_bytes(int prec, int op, int m, int n, int k, int batch) {
   return (size_t)batch * ns * m * n * c;
 }
 
+template <typename R, bool kTwoLevel>
+static void launch_tile(int op, dim3 grid, hipStream_t s, int m, int n, int k, int kchunk, int ns, const void* a,
+                        long long lda, long long sa, const void* b, long long ldb, long long sb, void* c,
+                        long long ldc, long long sc, double alpha, double beta, typename Cplx<R>::T* part) {
+  using C = typename Cplx<R>::T;
+  if (op == 0)
+    hipLaunchKernelGGL((k_zgemm<R, 0, kTwoLevel>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
+                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part);
+  else
+    hipLaunchKernelGGL((k_zgemm<R, 1, kTwoLevel>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
+                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part);
+}
+
 template <typename R>
 static int zgemm_t(int op, int m, int n, int k, double alpha, const void* a, long long lda, long long sa,
                    const void* b, long long ldb, long long sb, double beta, void* c, long long ldc, long long sc,
@@ -307,20 +319,15 @@ static int zgemm_t(int op, int m, int n, int k, double alpha, const void* a, lon
     return 1;
   }
   const dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM, batch * ns);
-  const bool two = std::is_same<R, float>::value && kchunk > kLongChunk;
   C* const p = ns > 1 ? part : nullptr;
-  if (op == 0 && two)
-    hipLaunchKernelGGL((k_zgemm<R, 0, true>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
-                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, p);
-  else if (op == 0)
-    hipLaunchKernelGGL((k_zgemm<R, 0, false>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
-                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, p);
-  else if (two)
-    hipLaunchKernelGGL((k_zgemm<R, 1, true>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
-                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, p);
-  else
-    hipLaunchKernelGGL((k_zgemm<R, 1, false>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
-                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, p);
+  if constexpr (std::is_same<R, float>::value) {
+    if (kchunk > kLongChunk)
+      launch_tile<R, true>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
+    else
+      launch_tile<R, false>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
+  } else {
+    launch_tile<R, false>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error(std::string("zgemm launch: ") + hipGetErrorString(e));

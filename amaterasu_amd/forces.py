@@ -120,18 +120,22 @@ def _datasets(system):
     return ds
 
 
+def _contract_g(w: torch.Tensor, side: _DensitySide) -> torch.Tensor:
+    """sum_G w[a, G] G for all atoms at once: one split-K MFMA product [natoms][ng] x [ng][3]
+    (a matrix-vector product per atom sends the long contraction to a single workgroup)."""
+    gt = side.g.T.to(torch.complex128).contiguous()  # [3][ng], real, so conj(gt)^T = G
+    return ops.zgemm(w.contiguous(), gt, 1)
+
+
 def local_forces(system, density) -> np.ndarray:
     side = _side(system)
     ds = _datasets(system)
     total = torch.as_tensor(np.asarray(density) if not isinstance(density, torch.Tensor) else density,
                             dtype=torch.float64).to("cuda").sum(0)
     ng = side.coefficients(total)
-    out = torch.zeros((len(system.symbols), 3), dtype=torch.complex128, device="cuda")
     ff = {s: side.shells(local_form_factor(d, side.uq)) for s, d in ds.items()}
-    for a, sym in enumerate(system.symbols):
-        w = 1j * ff[sym] * side.phase[a].conj() * ng
-        out[a] = -(w @ side.g.to(torch.complex128))
-    return _real(out, "local force")
+    w = torch.stack([ff[sym] for sym in system.symbols]) * side.phase.conj() * (1j * ng)[None, :]
+    return _real(-_contract_g(w, side), "local force")
 
 
 def core_forces(system, density) -> np.ndarray:
@@ -147,15 +151,11 @@ def core_forces(system, density) -> np.ndarray:
     xc = system.xc if isinstance(system.xc, XCFunctional) else XCFunctional(getattr(system.xc, "name", system.xc))
     _, vxc = xc.evaluate_on_grid(d + core_t[None] / d.shape[0], side.gops)
     vg = side.coefficients(vxc.mean(0))
-    out = torch.zeros((len(system.symbols), 3), dtype=torch.complex128, device="cuda")
-    for a, sym in enumerate(system.symbols):
-        dset = ds[sym]
-        if dset.core_density is None:
-            continue
-        cg = side.shells(density_form_factor(dset, dset.core_density, side.uq))
-        w = 1j * vg.conj() * cg * side.phase[a]
-        out[a] = w @ side.g.to(torch.complex128)
-    return _real(out, "core force")
+    zero = torch.zeros_like(side.g[:, 0])
+    cg = {s: zero if d.core_density is None else side.shells(density_form_factor(d, d.core_density, side.uq))
+          for s, d in ds.items()}
+    w = torch.stack([cg[sym] for sym in system.symbols]) * side.phase * (1j * vg.conj())[None, :]
+    return _real(_contract_g(w, side), "core force")
 
 
 def padded_block(wavefunctions, spin: int, ik: int, rows, n: int, npw_pad: int) -> torch.Tensor:
