@@ -67,14 +67,23 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
                                                long long sA, const typename Cplx<R>::T* __restrict__ B,
                                                long long ldb, long long sB, typename Cplx<R>::T* __restrict__ Cm,
                                                long long ldc, long long sC, R alpha, R beta,
-                                               typename Cplx<R>::T* __restrict__ part) {
+                                               typename Cplx<R>::T* __restrict__ part, int mfast) {
   using C = typename Cplx<R>::T;
   using acc_t = typename Mfma<R>::acc_t;
   __shared__ C As[BK][BM + 1];
   __shared__ C Bs[BK][BN + 1];
-  const int bz = blockIdx.z;
+  // XCD-aware order: workgroups are handed to the 8 XCDs round-robin by linear id, so
+  // logical tile t = (id % 8) * (T8 / 8) + id / 8 gives each XCD a contiguous run of tiles
+  // that share an operand panel in its L2: the N-tiles of one M-tile and K-chunk (A panel),
+  // or with mfast the M-tiles of one N-tile (B panel; NN products with few M-tiles).
+  const unsigned gx = gridDim.x, gy = gridDim.y, gxy = gridDim.x * gridDim.y;
+  const unsigned T = gxy * gridDim.z, T8 = T & ~7u;
+  const unsigned id = blockIdx.x + gx * blockIdx.y + gxy * blockIdx.z;
+  const unsigned tl = id < T8 ? (id & 7u) * (T8 >> 3) + (id >> 3) : id;
+  const unsigned txy = tl % gxy;
+  const int bz = (int)(tl / gxy);
   const int batch = bz / nsplit, split = bz % nsplit;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int m0 = (int)(mfast ? txy % gy : txy / gx) * BM, n0 = (int)(mfast ? txy / gy : txy % gx) * BN;
   const int kbeg = split * kchunk;
   const int kend = (kbeg + kchunk < K) ? kbeg + kchunk : K;
   A += (long long)batch * sA;
@@ -283,12 +292,13 @@ static void launch_tile(int op, dim3 grid, hipStream_t s, int m, int n, int k, i
                         long long lda, long long sa, const void* b, long long ldb, long long sb, void* c,
                         long long ldc, long long sc, double alpha, double beta, typename Cplx<R>::T* part) {
   using C = typename Cplx<R>::T;
+  const int mfast = op == 0 && grid.y > 1 && grid.x >= grid.y;
   if (op == 0)
     hipLaunchKernelGGL((k_zgemm<R, 0, kTwoLevel>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
-                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part);
+                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, mfast);
   else
     hipLaunchKernelGGL((k_zgemm<R, 1, kTwoLevel>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
-                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part);
+                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, 0);
 }
 
 template <typename R>
