@@ -118,6 +118,42 @@ def test_kbatch_matches_per_k_oracle(si_pseudo):
         assert np.all(out[k, :, len(s.g2):] == 0)  # padding stays zero
 
 
+def test_kbatch_fp32_band_pairs_across_k_points(si_pseudo):
+    """FP32 k-batch on the 42^3 grid with an odd band count (5 bands at each of 3 k-points,
+    so neighbouring band rows of the launch belong to different k-points, with different
+    stick maps and row lists). Against the oracle k by k, at the FP32 bound of the
+    single-k test."""
+    import torch
+
+    from amaterasu_amd.hamiltonian import Hamiltonian
+    from amaterasu_amd.system import build_system, load_dataset
+    from amaterasu_amd.workloads import seeded_coefficients, si8_kbatch, smooth_potential
+    from oracle.hamiltonian import Hamiltonian as OH
+    from oracle.system import build_system as obuild
+
+    spec = si8_kbatch(nk=3, nbands=5)
+    sysd = build_system(spec.cell, spec.positions, spec.symbols, {"Si": load_dataset()}, spec.ecut_rho,
+                        spec.kpoints, spec.ecuts)
+    syso = obuild(spec.cell, spec.positions, spec.symbols, {"Si": si_pseudo}, float(spec.ecuts[0]), spec.ecut_rho,
+                  spec.kpoints, wave_ecuts=spec.ecuts, with_ewald=False)
+    v = smooth_potential(sysd.grid.shape, seed=spec.seed)
+    hd = Hamiltonian(sysd, v, precision="fp32")
+    ho = OH(syso, v)
+    plan = hd.plan
+    psi = torch.zeros((3, 5, plan.npw_pad), dtype=torch.complex64, device="cuda")
+    blocks = []
+    for k, s in enumerate(sysd.wave_spheres):
+        c = seeded_coefficients(s.g2, 5, seed=200 + k)
+        blocks.append(c)
+        psi[k, :, : len(s.g2)] = torch.from_numpy(c).to("cuda", torch.complex64)
+    out = hd.apply_batched(psi).cpu().numpy()
+    for k, s in enumerate(sysd.wave_spheres):
+        ref = ho.apply(blocks[k], k)
+        got = out[k, :, : len(s.g2)]
+        assert np.abs(got - ref).max() / np.abs(ref).max() < 5e-6
+        assert np.all(out[k, :, len(s.g2):] == 0)
+
+
 def test_empty_lattice_free_electron_bands(si2_system):
     """Gate of test_hamiltonian.py:221-244 on the GPU path: V = 0, no projectors ->
     eigenvalues |k+G|^2/2 to 1e-11 Ha."""
