@@ -61,17 +61,23 @@ constexpr int BM = 64, BN = 64, BK = 16;
 constexpr int kFlushTiles = 4;
 constexpr int kLongChunk = 1024;
 
-template <typename R, int OP, bool kTwoLevel>
-__global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, int nsplit,
+// TM = 1: 64 x 64 tiles (each wavefront 32 x 32 as 2 x 2 MFMA blocks); TM = 2 (FP32, single
+// level): 128 x 128 tiles, each wavefront 64 x 64 as 4 x 4 blocks, half the operand traffic
+// per MFMA for large products.
+template <typename R, int OP, bool kTwoLevel, int TM>
+__global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, int K, int kchunk, int nsplit,
                                                const typename Cplx<R>::T* __restrict__ A, long long lda,
                                                long long sA, const typename Cplx<R>::T* __restrict__ B,
                                                long long ldb, long long sB, typename Cplx<R>::T* __restrict__ Cm,
                                                long long ldc, long long sC, R alpha, R beta,
                                                typename Cplx<R>::T* __restrict__ part, int mfast) {
+  static_assert(TM == 1 || !kTwoLevel, "the two-level accumulation uses 64 x 64 tiles");
   using C = typename Cplx<R>::T;
   using acc_t = typename Mfma<R>::acc_t;
-  __shared__ C As[BK][BM + 1];
-  __shared__ C Bs[BK][BN + 1];
+  constexpr int TBM = BM * TM, TBN = BN * TM, WB = 2 * TM;  // tile, MFMA blocks per wave side
+  constexpr int KQ = 4 * TM;                                 // consecutive k per thread in a row load
+  __shared__ C As[BK][TBM + 1];
+  __shared__ C Bs[BK][TBN + 1];
   // XCD-aware order: workgroups are handed to the 8 XCDs round-robin by linear id, so
   // logical tile t = (id % 8) * (T8 / 8) + id / 8 gives each XCD a contiguous run of tiles
   // that share an operand panel in its L2: the N-tiles of one M-tile and K-chunk (A panel),
@@ -83,71 +89,75 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
   const unsigned txy = tl % gxy;
   const int bz = (int)(tl / gxy);
   const int batch = bz / nsplit, split = bz % nsplit;
-  const int m0 = (int)(mfast ? txy % gy : txy / gx) * BM, n0 = (int)(mfast ? txy / gy : txy % gx) * BN;
+  const int m0 = (int)(mfast ? txy % gy : txy / gx) * TBM, n0 = (int)(mfast ? txy / gy : txy % gx) * TBN;
   const int kbeg = split * kchunk;
   const int kend = (kbeg + kchunk < K) ? kbeg + kchunk : K;
   A += (long long)batch * sA;
   B += (long long)batch * sB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  acc_t accr[2][2], acci[2][2];
+  const int wr = (wave >> 1) * 32 * TM, wc = (wave & 1) * 32 * TM;
+  acc_t accr[WB][WB], acci[WB][WB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < WB; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < WB; ++b)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         accr[a][b][q] = 0;
         acci[a][b][q] = 0;
       }
-  // second-level FP64 sums of the FP32 accumulators
-  double sumr[2][2][4], sumi[2][2][4];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        sumr[a][b][q] = 0.0;
-        sumi[a][b][q] = 0.0;
-      }
+  // second-level FP64 sums of the FP32 accumulators (two-level form only)
+  double sumr[kTwoLevel ? 2 : 1][kTwoLevel ? 2 : 1][4], sumi[kTwoLevel ? 2 : 1][kTwoLevel ? 2 : 1][4];
   auto flush = [&]() {
+    if constexpr (kTwoLevel) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            sumr[a][b][q] += (double)accr[a][b][q];
+            sumi[a][b][q] += (double)acci[a][b][q];
+            accr[a][b][q] = 0;
+            acci[a][b][q] = 0;
+          }
+    }
+  };
+  if constexpr (kTwoLevel) {
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          sumr[a][b][q] += (double)accr[a][b][q];
-          sumi[a][b][q] += (double)acci[a][b][q];
-          accr[a][b][q] = 0;
-          acci[a][b][q] = 0;
+          sumr[a][b][q] = 0.0;
+          sumi[a][b][q] = 0.0;
         }
-  };
+  }
   int tiles = 0;
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
     {
-      const int row = tid >> 2, kq = (tid & 3) * 4;
+      const int row = tid / (BK / KQ), kq = (tid % (BK / KQ)) * KQ;
       const int gm = m0 + row;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < KQ; ++u) {
         const int gk = k0 + kq + u;
         As[kq + u][row] = (gm < M && gk < kend) ? A[(long long)gm * lda + gk] : cmake<C>(0, 0);
       }
     }
     if (OP == 0) {
-      const int kk = tid >> 4, c0 = (tid & 15) * 4;
+      const int kk = tid >> 4, c0 = (tid & 15) * KQ;
       const int gk = k0 + kk;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < KQ; ++u) {
         const int gn = n0 + c0 + u;
         Bs[kk][c0 + u] = (gk < kend && gn < N) ? B[(long long)gk * ldb + gn] : cmake<C>(0, 0);
       }
     } else {
-      const int col = tid >> 2, kq = (tid & 3) * 4;
+      const int col = tid / (BK / KQ), kq = (tid % (BK / KQ)) * KQ;
       const int gn = n0 + col;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < KQ; ++u) {
         const int gk = k0 + kq + u;
         C v = (gn < N && gk < kend) ? B[(long long)gn * ldb + gk] : cmake<C>(0, 0);
         v.y = -v.y;
@@ -158,15 +168,15 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4);
-      C a[2], b[2];
+      C a[WB], b[WB];
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) a[rb] = As[kr][wr + rb * 16 + (lane & 15)];
+      for (int rb = 0; rb < WB; ++rb) a[rb] = As[kr][wr + rb * 16 + (lane & 15)];
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb) b[cb] = Bs[kr][wc + cb * 16 + (lane & 15)];
+      for (int cb = 0; cb < WB; ++cb) b[cb] = Bs[kr][wc + cb * 16 + (lane & 15)];
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
+      for (int rb = 0; rb < WB; ++rb)
 #pragma unroll
-        for (int cb = 0; cb < 2; ++cb) {
+        for (int cb = 0; cb < WB; ++cb) {
           accr[rb][cb] = Mfma<R>::mma(a[rb].x, b[cb].x, accr[rb][cb]);
           accr[rb][cb] = Mfma<R>::mma(-a[rb].y, b[cb].y, accr[rb][cb]);
           acci[rb][cb] = Mfma<R>::mma(a[rb].x, b[cb].y, acci[rb][cb]);
@@ -181,18 +191,21 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, int kchunk, 
       }
     }
   }
-  if constexpr (kTwoLevel) flush();
+  flush();
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
+  for (int rb = 0; rb < WB; ++rb)
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
+    for (int cb = 0; cb < WB; ++cb)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = m0 + wr + rb * 16 + Mfma<R>::row(lane, q);
         const int col = n0 + wc + cb * 16 + (lane & 15);
         if (row < M && col < N) {
-          const C v = kTwoLevel ? cmake<C>((R)sumr[rb][cb][q], (R)sumi[rb][cb][q])
-                                : cmake<C>(accr[rb][cb][q], acci[rb][cb][q]);
+          C v;
+          if constexpr (kTwoLevel)
+            v = cmake<C>((R)sumr[rb][cb][q], (R)sumi[rb][cb][q]);
+          else
+            v = cmake<C>(accr[rb][cb][q], acci[rb][cb][q]);
           if (part) {
             part[(((long long)batch * nsplit + split) * M + row) * N + col] = v;
           } else {
@@ -279,26 +292,57 @@ int set_zgemm_split(int prec, int op, int m, int n, int k, int batch, int split)
   return 0;
 }
 
-size_t zgemm_workspace_bytes(int prec, int op, int m, int n, int k, int batch) {
-  (void)op;
-  const int ns = choose_split(prec, op, m, n, k, batch);
-  if (ns <= 1) return 0;
-  const size_t c = prec == kFP64 ? 16 : 8;
-  return (size_t)batch * ns * m * n * c;
+// Large FP32 NC products (overlaps and projections: both output sides >= kBigSide, K the
+// plane waves) use 128 x 128 tiles with K chunks of at most kBigChunk: one FP32
+// accumulator per chunk (as accurate as the two-level form for these lengths) and the
+// FP64 split reduction, at the cost of writing and reading the partial products once per
+// chunk (about 5 % of the product's time). Measured on the Si214 overlaps 1.4-2x faster
+// than 64 x 64 tiles; NN products (rotations, projector application) were not faster and
+// keep 64 x 64. Beyond kBigWorkspace of partials the two-level form is used instead.
+constexpr int kBigSide = 256;
+constexpr int kBigChunk = 512;
+constexpr size_t kBigWorkspace = size_t(24) << 30;
+
+struct GemmPlan {
+  int tm;  // tile multiplier
+  int ns;  // K splits (before rounding the chunk to BK)
+};
+
+static GemmPlan plan_gemm(int prec, int op, int m, int n, int k, int batch) {
+  if (prec == kFP32 && op == 1 && m >= kBigSide && n >= kBigSide && k > 0) {
+    const long long tiles = (long long)((m + 2 * BM - 1) / (2 * BM)) * ((n + 2 * BN - 1) / (2 * BN)) * batch;
+    long long ns = (k + kBigChunk - 1) / kBigChunk;
+    if (tiles < 512) {
+      long long want = (1024 + tiles - 1) / tiles;
+      const long long kmax = (k + 255) / 256;
+      if (want > kmax) want = kmax;
+      if (want > ns) ns = want;
+    }
+    const size_t bytes = ns > 1 ? (size_t)batch * ns * m * n * 8 : 0;
+    if (batch * ns <= 65535 && bytes <= kBigWorkspace) return {2, (int)ns};
+  }
+  return {1, choose_split(prec, op, m, n, k, batch)};
 }
 
-template <typename R, bool kTwoLevel>
+size_t zgemm_workspace_bytes(int prec, int op, int m, int n, int k, int batch) {
+  const GemmPlan g = plan_gemm(prec, op, m, n, k, batch);
+  if (g.ns <= 1) return 0;
+  const size_t c = prec == kFP64 ? 16 : 8;
+  return (size_t)batch * g.ns * m * n * c;
+}
+
+template <typename R, bool kTwoLevel, int TM>
 static void launch_tile(int op, dim3 grid, hipStream_t s, int m, int n, int k, int kchunk, int ns, const void* a,
                         long long lda, long long sa, const void* b, long long ldb, long long sb, void* c,
                         long long ldc, long long sc, double alpha, double beta, typename Cplx<R>::T* part) {
   using C = typename Cplx<R>::T;
   const int mfast = op == 0 && grid.y > 1 && grid.x >= grid.y;
   if (op == 0)
-    hipLaunchKernelGGL((k_zgemm<R, 0, kTwoLevel>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
-                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, mfast);
+    hipLaunchKernelGGL((k_zgemm<R, 0, kTwoLevel, TM>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda,
+                       sa, (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, mfast);
   else
-    hipLaunchKernelGGL((k_zgemm<R, 1, kTwoLevel>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda, sa,
-                       (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, 0);
+    hipLaunchKernelGGL((k_zgemm<R, 1, kTwoLevel, TM>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda,
+                       sa, (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, 0);
 }
 
 template <typename R>
@@ -307,8 +351,8 @@ static int zgemm_t(int op, int m, int n, int k, double alpha, const void* a, lon
                    int batch, void* ws, size_t ws_bytes, hipStream_t s) {
   using C = typename Cplx<R>::T;
   if (m <= 0 || n <= 0 || batch <= 0) return 0;
-  int ns = choose_split(std::is_same<R, double>::value ? kFP64 : kFP32, op, m, n, k, batch);
-  if (k <= 0) ns = 1;
+  const GemmPlan g = plan_gemm(std::is_same<R, double>::value ? kFP64 : kFP32, op, m, n, k, batch);
+  int ns = k <= 0 ? 1 : g.ns;
   const size_t need = ns > 1 ? (size_t)batch * ns * m * n * sizeof(C) : 0;
   C* part = nullptr;
   if (ns > 1) {
@@ -328,15 +372,18 @@ static int zgemm_t(int op, int m, int n, int k, double alpha, const void* a, lon
     set_error("zgemm: batch x split exceeds the grid limit");
     return 1;
   }
-  const dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM, batch * ns);
+  const int tb = g.tm == 2 ? 2 * BM : BM;
+  const dim3 grid((n + tb - 1) / tb, (m + tb - 1) / tb, batch * ns);
   C* const p = ns > 1 ? part : nullptr;
   if constexpr (std::is_same<R, float>::value) {
-    if (kchunk > kLongChunk)
-      launch_tile<R, true>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
+    if (g.tm == 2)
+      launch_tile<R, false, 2>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
+    else if (kchunk > kLongChunk)
+      launch_tile<R, true, 1>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
     else
-      launch_tile<R, false>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
+      launch_tile<R, false, 1>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
   } else {
-    launch_tile<R, false>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
+    launch_tile<R, false, 1>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {

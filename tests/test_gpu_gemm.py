@@ -15,7 +15,8 @@ def _c(shape, seed):
 
 @pytest.mark.parametrize("dtype,tol", [("complex128", 1e-13), ("complex64", 2e-5)])
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 9), (64, 64, 16), (100, 70, 300), (8, 36, 754),
-                                   (160, 1152, 3000), (37, 5, 40000)])
+                                   (160, 1152, 3000), (37, 5, 40000), (300, 270, 1000), (257, 513, 700),
+                                   (256, 384, 3000)])
 def test_zgemm_nn_and_nc(dtype, tol, m, n, k):
     import torch
 
@@ -107,3 +108,26 @@ def test_zgemm_fp32_long_contraction_accumulates_in_fp64(split):
     want = ops.zgemm(a64, a64, 1)
     assert float((torch.diagonal(want) - 1.0).abs().max()) < 1e-6
     assert float((got - want).abs().max()) < 3e-7
+
+
+def test_zgemm_fp32_large_tiles_long_contraction():
+    """FP32 NC products with both output sides >= 256 take 128 x 128 tiles with K chunks of
+    512 and the FP64 split reduction: the same K = 131 072 unit-row Gram check (batched, so
+    the batch offsets of the partial products are exercised too), and an NN rotation of the
+    same rows."""
+    import torch
+
+    from amaterasu_amd import ops
+
+    nb, m, k = 2, 256, 131072
+    g = torch.Generator().manual_seed(10)
+    a = torch.randn((nb, m, k), dtype=torch.complex128, generator=g)
+    a = (a / torch.linalg.vector_norm(a, dim=-1, keepdim=True)).to(torch.complex64).cuda()
+    a64 = a.to(torch.complex128)
+    got = ops.zgemm(a, a, 1).to(torch.complex128)
+    want = ops.zgemm(a64, a64, 1)
+    assert float((got - want).abs().max()) < 3e-7
+    rot = torch.randn((nb, 300, m), dtype=torch.complex128, generator=g).cuda() / m**0.5
+    got = ops.zgemm(rot.to(torch.complex64), a, 0).to(torch.complex128)
+    want = ops.zgemm(rot.to(torch.complex64).to(torch.complex128), a64, 0)
+    assert float((got - want).abs().max() / want.abs().max()) < 1e-6
