@@ -25,6 +25,8 @@ import torch
 from scipy.integrate import simpson
 from scipy.special import spherical_jn
 
+from . import ops
+
 MAX_L = 3
 _XC_NAMES = {2: "LDA-PZ", 7: "LDA-PW", 11: "PBE", -116133: "PBEsol", -101130: "PBE", -1012: "LDA-PW"}
 
@@ -218,6 +220,76 @@ class Projectors:
 
     def host_matrix(self) -> np.ndarray:
         return self.matrix.cpu().numpy()
+
+    # The operator methods of ``KBProjectors`` (``pseudo/kb.py:216-260``) on the device
+    # GEMMs. numpy in gives numpy out; device tensors stay on the device (FP64, or FP32
+    # for complex64 input in ``project`` and ``apply``).
+    def _block(self, coefficients):
+        host = not isinstance(coefficients, torch.Tensor)
+        c = torch.from_numpy(np.asarray(coefficients, dtype=complex)) if host else coefficients
+        if c.shape[-1] != self.matrix.shape[-1]:
+            raise ValueError(f"coefficients have {c.shape[-1]} plane waves, the projectors {self.matrix.shape[-1]}")
+        one = c.dim() == 1
+        c2 = (c[None, :] if one else c.reshape(-1, c.shape[-1])).to("cuda")
+        if c2.dtype not in (torch.complex64, torch.complex128):
+            c2 = c2.to(torch.complex128)
+        return c2.contiguous(), host, one, c.shape
+
+    def _matrix(self, dtype) -> torch.Tensor:
+        return self.matrix.to("cuda", dtype).contiguous()
+
+    def _coupling(self, coupling, dtype) -> torch.Tensor:
+        d = self.dij if coupling is None else coupling
+        return torch.as_tensor(np.asarray(d) if not isinstance(d, torch.Tensor) else d).to("cuda", dtype)
+
+    def project(self, coefficients):
+        """``<beta_i|psi_n>`` for a band or a block of bands, shape ``(..., nproj)``."""
+        c2, host, one, shape = self._block(coefficients)
+        p = ops.zgemm(c2, self._matrix(c2.dtype), 1)
+        p = p[0] if one else p.reshape(*shape[:-1], p.shape[-1])
+        return p.cpu().numpy() if host else p
+
+    def apply(self, coefficients, coupling=None):
+        """``V_NL |psi>``, same shape as ``coefficients``; ``coupling`` overrides D_ij."""
+        c2, host, one, shape = self._block(coefficients)
+        if len(self.labels) == 0:
+            out = torch.zeros_like(c2)
+        else:
+            p = ops.zgemm(c2, self._matrix(c2.dtype), 1)
+            pd = (p @ self._coupling(coupling, c2.dtype)).contiguous()
+            out = ops.zgemm(pd, self._matrix(c2.dtype), 0)
+        out = out[0] if one else out.reshape(shape)
+        return out.cpu().numpy() if host else out
+
+    def _projections64(self, coefficients) -> torch.Tensor:
+        c2, _, _, _ = self._block(coefficients)
+        c2 = c2.to(torch.complex128)
+        return ops.zgemm(c2, self._matrix(torch.complex128), 1)
+
+    def energy(self, coefficients, weights, coupling=None) -> float:
+        """``sum_n w_n <psi_n|V_NL|psi_n>`` in Hartree (FP64)."""
+        if len(self.labels) == 0:
+            return 0.0
+        p = self._projections64(coefficients)
+        w = torch.as_tensor(np.asarray(weights, dtype=float) if not isinstance(weights, torch.Tensor) else weights,
+                            dtype=torch.float64).to("cuda").reshape(-1)
+        quad = (p.conj() * (p @ self._coupling(coupling, torch.complex128))).sum(-1).real
+        return float((w * quad).sum().item())
+
+    def occupancies(self, coefficients, weights) -> np.ndarray:
+        """``D_ij = Re sum_n w_n <p_i|psi_n><psi_n|p_j>`` (FP64, numpy)."""
+        if len(self.labels) == 0:
+            return np.zeros((0, 0))
+        p = self._projections64(coefficients)
+        w = torch.as_tensor(np.asarray(weights, dtype=float) if not isinstance(weights, torch.Tensor) else weights,
+                            dtype=torch.float64).to("cuda").reshape(-1)
+        occ = (p * w[:, None]).T.contiguous() @ p.conj()
+        return occ.real.cpu().numpy()
+
+    def completeness(self) -> np.ndarray:
+        """``sum_G |<k+G|beta_i>|^2`` per projector (FP64, numpy)."""
+        m = self._matrix(torch.complex128)
+        return (m.real**2 + m.imag**2).sum(-1).cpu().numpy()
 
 
 def build_projectors(sphere, positions, symbols, datasets: dict, volume: float, *, device="cuda") -> Projectors:

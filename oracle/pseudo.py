@@ -359,7 +359,15 @@ class KBProjectors:
         p = self.project(c)
         return float(np.dot(weights, np.einsum("ni,ij,nj->n", p.conj(), d, p).real))
 
+    def occupancies(self, c, weights) -> np.ndarray:
+        """``kb.py:243-256``: Re sum_n w_n <p_i|psi_n><psi_n|p_j>."""
+        if len(self.labels) == 0:
+            return np.zeros((0, 0))
+        p = self.project(c)
+        return np.einsum("n,ni,nj->ij", weights, p, p.conj()).real
+
     def completeness(self) -> np.ndarray:
+        """``kb.py:258-260``: sum_G |<k+G|beta_i>|^2."""
         return np.einsum("ig,ig->i", self.matrix.conj(), self.matrix).real
 
 

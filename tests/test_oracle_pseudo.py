@@ -134,6 +134,27 @@ def test_projector_operator_is_hermitian(si_pseudo):
     assert m == pytest.approx(m.conj().T, abs=1e-12)
 
 
+def test_projector_occupancies_against_the_energy(si_pseudo):
+    """occupancies (kb.py:243-256) through the energy it must reproduce: for a real
+    symmetric D, sum_n w_n <psi_n|V_NL|psi_n> = trace(D occ); occ is symmetric and, for
+    non-negative weights, positive semi-definite; a single band with weight 1 gives the
+    outer product of its projections."""
+    cell, pos = si_primitive_bohr()
+    s = GSphere.build(cell, 8.0, kpoint=(0.1, -0.2, 0.3))
+    kb = build_projectors(s, pos, ["Si", "Si"], {"Si": si_pseudo}, abs(np.linalg.det(cell)))
+    rng = np.random.default_rng(4)
+    b = rng.normal(size=(5, len(s))) + 1j * rng.normal(size=(5, len(s)))
+    w = np.array([1.0, 0.5, 0.25, 0.0, 2.0])
+    occ = kb.occupancies(b, w)
+    assert occ.shape == (len(kb), len(kb))
+    assert np.abs(occ - occ.T).max() < 1e-12 * np.abs(occ).max()
+    assert np.linalg.eigvalsh(occ).min() > -1e-10 * np.abs(occ).max()
+    assert np.trace(kb.dij @ occ) == pytest.approx(kb.energy(b, w), rel=1e-12)
+    one = kb.occupancies(b[:1], np.ones(1))
+    p = kb.project(b[0])
+    assert np.abs(one - np.outer(p, p.conj()).real).max() < 1e-12 * np.abs(one).max()
+
+
 def test_projector_element_closed_form(si_pseudo):
     """One matrix element recomputed from its definition
     (4 pi/sqrt Omega)(-i)^l Y_lm f_l e^{-i(k+G).tau} with an independent quadrature of f_l."""
