@@ -116,9 +116,22 @@ class FFTGrid:
     def cell_volume_per_point(self) -> float:
         return self.volume / self.size
 
+    def __str__(self) -> str:
+        n1, n2, n3 = self.shape
+        return f"{n1}x{n2}x{n3} = {self.size} points ({self.backend} backend)"
+
     @classmethod
-    def for_cutoff(cls, cell, ecut: float, *, multiple: int = 1) -> "FFTGrid":
-        return cls(shape=grid_shape_for_cutoff(cell, ecut, multiple), cell=np.asarray(cell, dtype=float))
+    def for_cutoff(cls, cell, ecut: float, *, backend: str = "hip", multiple: int = 1) -> "FFTGrid":
+        """``fft.py:151-162``: smallest efficient grid holding the ecut sphere."""
+        return cls(shape=grid_shape_for_cutoff(cell, ecut, multiple), cell=np.asarray(cell, dtype=float),
+                   backend=backend)
+
+    @classmethod
+    def for_sphere(cls, sphere, *, backend: str = "hip", multiple: int = 1) -> "FFTGrid":
+        """``fft.py:164-172``: smallest efficient grid holding ``sphere`` without aliasing."""
+        hmax = np.abs(sphere.miller).max(axis=0)
+        return cls(shape=tuple(fast_length(int(2 * h + 1), multiple) for h in hmax),
+                   cell=np.asarray(sphere.cell, dtype=float), backend=backend)
 
     def indices(self, sphere):
         """``fft.py:174-190``."""
@@ -196,6 +209,45 @@ class FFTGrid:
 
     def points_cartesian(self) -> np.ndarray:
         return self.points_fractional() @ self.cell
+
+    def gvectors_cartesian(self, *, zero_nyquist: bool = False) -> np.ndarray:
+        """``fft.py:254-286``: Cartesian G of every grid point (FFT frequency order), 1/Bohr,
+        shape ``(*shape, 3)``; ``zero_nyquist`` blanks the n/2 component of even axes."""
+        axes = []
+        for n in self.shape:
+            f = np.fft.fftfreq(n, d=1.0 / n)
+            if zero_nyquist and n % 2 == 0:
+                f[n // 2] = 0.0
+            axes.append(f)
+        miller = np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1)
+        return miller @ reciprocal_cell(self.cell)
+
+    def _ig(self) -> torch.Tensor:
+        """i G (Nyquist blanked) on the device, shape (3, *shape), complex128 (cached)."""
+        ig = self._plans.get("_ig")
+        if ig is None:
+            g = torch.from_numpy(np.ascontiguousarray(np.moveaxis(self.gvectors_cartesian(zero_nyquist=True), -1, 0)))
+            ig = (1j * g).to("cuda", torch.complex128)
+            self._plans["_ig"] = ig
+        return ig
+
+    def gradient(self, values):
+        """``fft.py:288-295``: gradient of a real periodic field via i G, shape ``(*shape, 3)``."""
+        host = _host(values)
+        v = torch.as_tensor(np.asarray(values) if host else values).to("cuda", torch.complex128)
+        c = ops.fft3d(v.reshape(self.shape).contiguous(), -1, 1.0 / self.size)
+        d = ops.fft3d((self._ig() * c[None]).contiguous(), +1, 1.0, inplace=True).real
+        out = d.permute(1, 2, 3, 0).contiguous()
+        return out.cpu().numpy() if host else out
+
+    def divergence(self, field):
+        """``fft.py:297-304``: divergence of a real vector field of shape ``(*shape, 3)``."""
+        host = _host(field)
+        f = torch.as_tensor(np.asarray(field) if host else field).to("cuda", torch.complex128)
+        c = ops.fft3d(f.permute(3, 0, 1, 2).contiguous(), -1, 1.0 / self.size, inplace=True)
+        acc = (self._ig() * c).sum(0).contiguous()
+        out = ops.fft3d(acc, +1, 1.0, inplace=True).real.contiguous()
+        return out.cpu().numpy() if host else out
 
     def integrate(self, values):
         """``fft.py:306-308``."""

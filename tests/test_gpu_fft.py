@@ -81,3 +81,31 @@ def test_fftgrid_drop_in_matches_the_oracle(si2_system):
     assert np.abs(back - c).max() < 1e-13  # test_fft.py:49-55 round trip
     with pytest.raises(ValueError, match="alias"):
         FFTGrid((8, 8, 8), o.cell).scatter(np.ones(len(si2_system.density_sphere)), si2_system.density_sphere)
+
+
+def test_fftgrid_gradient_divergence_and_constructors_match_the_oracle(si2_system):
+    """FFTGrid.gradient/divergence/gvectors_cartesian/for_sphere (basis/fft.py:164-304) on
+    the GPU against the oracle's, numpy in and out and device in and out."""
+    import torch
+
+    from amaterasu_amd.basis import FFTGrid
+    from oracle.basis import FFTGrid as OFFTGrid
+
+    o = si2_system.grid
+    g = FFTGrid(o.shape, o.cell)
+    assert str(g).startswith(f"{o.shape[0]}x{o.shape[1]}x{o.shape[2]} = {g.size} points")
+    sphere = si2_system.density_sphere
+    assert FFTGrid.for_sphere(sphere).shape == OFFTGrid.for_sphere(sphere).shape
+    assert np.array_equal(g.gvectors_cartesian(zero_nyquist=True), o.gvectors_cartesian(zero_nyquist=True))
+    rng = np.random.default_rng(8)
+    u = rng.normal(size=o.shape)
+    ref = o.gradient(u)
+    got = g.gradient(u)
+    assert got.shape == ref.shape and np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    f = rng.normal(size=(*o.shape, 3))
+    ref = o.divergence(f)
+    got = g.divergence(f)
+    assert got.shape == ref.shape and np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    dev = g.gradient(torch.from_numpy(u).cuda())
+    assert isinstance(dev, torch.Tensor) and dev.is_cuda
+    assert np.abs(dev.cpu().numpy() - o.gradient(u)).max() < 1e-12 * np.abs(o.gradient(u)).max()

@@ -134,3 +134,28 @@ def test_alias_raises(si_cell_bohr, spheres_and_grid):
     _, rho, _ = spheres_and_grid
     with pytest.raises(ValueError, match="alias"):
         FFTGrid(shape=(8, 8, 8), cell=si_cell_bohr).scatter(np.ones(len(rho)), rho)
+
+
+def test_gradient_and_divergence_of_a_plane_wave():
+    """fft.py:288-304 on a triclinic cell: for u = cos(G.r) with G a grid vector below
+    Nyquist, grad u = -G sin(G.r) and div grad u = -|G|^2 cos(G.r); a field with weight on
+    the Nyquist plane of an even axis loses it in the gradient (zero_nyquist), while
+    gvectors_cartesian without blanking keeps -n/2 there."""
+    from oracle.basis import FFTGrid, reciprocal_cell
+
+    cell = np.array([[5.1, 0.2, -0.3], [0.4, 4.7, 0.1], [-0.2, 0.3, 6.2]])
+    grid = FFTGrid((12, 10, 15), cell)
+    h = np.array([2, -3, 4])
+    gvec = h @ reciprocal_cell(cell)
+    r = grid.points_cartesian()
+    phase = r @ gvec
+    u = np.cos(phase)
+    grad = grid.gradient(u)
+    assert np.abs(grad + gvec * np.sin(phase)[..., None]).max() < 1e-11
+    lap = grid.divergence(grad)
+    assert np.abs(lap + (gvec @ gvec) * u).max() < 1e-10
+    nyq = np.cos(np.pi * np.arange(12))[:, None, None] * np.ones((12, 10, 15))
+    assert np.abs(grid.gradient(nyq)).max() < 1e-12
+    g_full = grid.gvectors_cartesian()
+    assert np.allclose(g_full[6, 0, 0], -6 * reciprocal_cell(cell)[0])
+    assert np.allclose(grid.gvectors_cartesian(zero_nyquist=True)[6, 0, 0], 0.0)
