@@ -337,12 +337,16 @@ static void launch_tile(int op, dim3 grid, hipStream_t s, int m, int n, int k, i
                         long long ldc, long long sc, double alpha, double beta, typename Cplx<R>::T* part) {
   using C = typename Cplx<R>::T;
   const int mfast = op == 0 && grid.y > 1 && grid.x >= grid.y;
-  if (op == 0)
-    hipLaunchKernelGGL((k_zgemm<R, 0, kTwoLevel, TM>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda,
-                       sa, (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, mfast);
-  else
-    hipLaunchKernelGGL((k_zgemm<R, 1, kTwoLevel, TM>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda,
-                       sa, (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, 0);
+  if constexpr (TM == 1) {
+    if (op == 0) {
+      hipLaunchKernelGGL((k_zgemm<R, 0, kTwoLevel, 1>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a,
+                         lda, sa, (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, mfast);
+      return;
+    }
+  }
+  // 128 x 128 tiles are planned for NC products only
+  hipLaunchKernelGGL((k_zgemm<R, 1, kTwoLevel, TM>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda,
+                     sa, (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, 0);
 }
 
 template <typename R>
