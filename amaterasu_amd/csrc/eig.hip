@@ -177,7 +177,7 @@ __device__ __forceinline__ double2 block_sum2_once(double2 v, double* red) {
 // sum p^H v, the publication of w, and the norm of the new column. The reflector of column
 // j+1 (zlarfg) is kept unscaled in LDS; its scale enters the product with the owned row
 // once, after the row's dot product, and the stored V row.
-// kRowRegs = elements of one row per lane: 16 up to n = 1024, 32 up to n = 2048.
+// kRowRegs = elements of one row per lane: 16, for n up to 1024 (larger n: k_hetrd_step_wide).
 
 template <int kRowRegs>
 __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A, int n, int j,
@@ -313,6 +313,165 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A,
         a0 = rowv[q];
       } else if (c > j + 2 && c < n) {
         const double2 xv = sx[c - j - 2];
+        a1.x += rowv[q].x * xv.x - rowv[q].y * xv.y;
+        a1.y += rowv[q].x * xv.y + rowv[q].y * xv.x;
+      }
+    }
+    const double2 s1 = zmul(sc, a1);
+    double2 acc = make_double2(a0.x + s1.x, a0.y + s1.y);
+    for (int off = 32; off > 0; off >>= 1) {
+      acc.x += __shfl_down(acc.x, off, 64);
+      acc.y += __shfl_down(acc.y, off, 64);
+    }
+    if (lane == 0) p_out[myrow - j - 2] = zmul(tau2, acc);
+  }
+}
+
+// The step for 1024 < n <= 2048: 32 row elements per lane take 128 VGPRs, so v and w live
+// in LDS only (not also in registers) and the new column reuses v's LDS after one more
+// barrier: at most 256 registers and 2n elements of LDS per workgroup (56 KB at n = 1760),
+// which lets two workgroups share a CU. With three vectors in LDS and in registers (the
+// n <= 1024 form) one workgroup per CU fit, and a 440-workgroup launch ran in two rounds.
+__global__ __launch_bounds__(eig::NT, 2) void k_hetrd_step_wide(double2* __restrict__ A, int n, int j,
+                                                               double2* __restrict__ V, double2* __restrict__ taus,
+                                                               double* __restrict__ d, double* __restrict__ e,
+                                                               const double2* __restrict__ p_in,
+                                                               double2* __restrict__ p_out) {
+  constexpr int RR = eig::MAXN / 64;      // row elements per lane
+  constexpr int QV = eig::MAXN / eig::NT;  // vector elements per thread
+  double2* sv = hetrd_dyn;      // v_j, then the column j+1 below the diagonal (unscaled)
+  double2* sw = hetrd_dyn + n;  // p_j, then w_j
+  __shared__ double red[2][2 * (eig::NT / 64)];
+  const int len = n - j - 1;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const double2 zero = make_double2(0.0, 0.0);
+  const int myrow = j + 2 + blockIdx.x + wave * gridDim.x;
+  const bool own = myrow < n;
+  double2 rowv[RR];
+#pragma unroll
+  for (int q = 0; q < RR; ++q) {
+    const int c = j + 1 + lane + 64 * q;
+    rowv[q] = (own && c < n) ? A[(long long)myrow * n + c] : make_double2(0.0, 0.0);
+  }
+  double2 piv[QV];
+#pragma unroll
+  for (int q = 0; q < QV; ++q) {
+    const int t = tid + eig::NT * q;
+    piv[q] = t < len ? A[(long long)(j + 1) * n + j + 1 + t] : make_double2(0.0, 0.0);
+    if (t < len) {
+      sv[t] = V[(long long)j * n + t];
+      sw[t] = p_in[t];
+    }
+  }
+  const double2 tau = taus[j];
+  // K = -(tau/2) p^H v; w = p + K v (each thread on its own elements)
+  double2 part = zero;
+#pragma unroll
+  for (int q = 0; q < QV; ++q) {
+    const int t = tid + eig::NT * q;
+    if (t < len) {
+      const double2 pv = zmul(zconj(sw[t]), sv[t]);
+      part.x += pv.x;
+      part.y += pv.y;
+    }
+  }
+  const double2 phv = block_sum2_once(part, red[0]);
+  const double2 K = zmul(make_double2(-0.5 * tau.x, -0.5 * tau.y), phv);
+#pragma unroll
+  for (int q = 0; q < QV; ++q) {
+    const int t = tid + eig::NT * q;
+    if (t < len) {
+      const double2 kv = zmul(K, sv[t]);
+      sw[t] = make_double2(sw[t].x + kv.x, sw[t].y + kv.y);
+    }
+  }
+  __syncthreads();
+  if (own) {
+    const int tr = myrow - j - 1;
+    const double2 vr = sv[tr], wr = sw[tr];
+    double2* row = A + (long long)myrow * n;
+#pragma unroll
+    for (int q = 0; q < RR; ++q) {
+      const int c = j + 1 + lane + 64 * q;
+      if (c < n) {
+        const int tc = c - j - 1;
+        const double2 a = zmul(vr, zconj(sw[tc]));
+        const double2 b = zmul(wr, zconj(sv[tc]));
+        rowv[q].x -= a.x + b.x;
+        rowv[q].y -= a.y + b.y;
+        row[c] = rowv[q];
+      }
+    }
+  }
+  const double2 v0 = sv[0], w0 = sw[0];
+  const int len2 = n - j - 2;
+  if (len2 <= 0) {
+    if (blockIdx.x == 0 && tid == 0) {
+      const double2 a = zmul(v0, zconj(w0));
+      d[j + 1] = piv[0].x - 2.0 * a.x;
+    }
+    return;
+  }
+  double2 xq[QV];
+  double nrm2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < QV; ++q) {
+    const int t = tid + eig::NT * q;
+    xq[q] = zero;
+    if (t < len) {
+      const double2 a = zmul(v0, zconj(sw[t]));
+      const double2 b = zmul(w0, zconj(sv[t]));
+      const double2 nr = make_double2(piv[q].x - a.x - b.x, piv[q].y - a.y - b.y);
+      xq[q] = zconj(nr);
+      if (t == 0) {
+        if (blockIdx.x == 0) d[j + 1] = nr.x;
+      } else if (t >= 2) {
+        nrm2 += nr.x * nr.x + nr.y * nr.y;
+      }
+    }
+  }
+  __syncthreads();  // every wave has finished reading v: its LDS becomes the new column
+#pragma unroll
+  for (int q = 0; q < QV; ++q) {
+    const int t = tid + eig::NT * q;
+    if (t >= 1 && t < len) sv[t - 1] = xq[q];
+  }
+  const double xnorm2 = block_sum2_once(make_double2(nrm2, 0.0), red[1]).x;
+  const double2 alpha = sv[0];
+  double beta;
+  double2 tau2, sc;
+  if (xnorm2 == 0.0 && alpha.y == 0.0) {
+    beta = alpha.x;
+    tau2 = zero;
+    sc = zero;
+  } else {
+    const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
+    beta = alpha.x >= 0.0 ? -nrm : nrm;
+    tau2 = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+    const double2 dd = make_double2(alpha.x - beta, alpha.y);
+    const double den = dd.x * dd.x + dd.y * dd.y;
+    sc = make_double2(dd.x / den, -dd.y / den);
+  }
+  if (blockIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < QV; ++q) {
+      const int t = tid + eig::NT * q;
+      if (t >= 1 && t < len) V[(long long)(j + 1) * n + t - 1] = t == 1 ? make_double2(1.0, 0.0) : zmul(xq[q], sc);
+    }
+    if (tid == 0) {
+      taus[j + 1] = tau2;
+      e[j + 1] = beta;
+    }
+  }
+  if (own) {
+    double2 a0 = zero, a1 = zero;
+#pragma unroll
+    for (int q = 0; q < RR; ++q) {
+      const int c = j + 1 + lane + 64 * q;
+      if (c == j + 2) {
+        a0 = rowv[q];
+      } else if (c > j + 2 && c < n) {
+        const double2 xv = sv[c - j - 2];
         a1.x += rowv[q].x * xv.x - rowv[q].y * xv.y;
         a1.y += rowv[q].x * xv.y + rowv[q].y * xv.x;
       }
@@ -787,8 +946,8 @@ static void enqueue_tridiag(hipStream_t s, int n, double2* A, double2* V, double
       hipLaunchKernelGGL(k_hetrd_step<eig::MAXN / 128>, dim3(G), dim3(eig::NT), 3 * (size_t)n * sizeof(double2), s,
                          A, n, j, V, taus, d, e, pin, pout);
     else
-      hipLaunchKernelGGL(k_hetrd_step<eig::MAXN / 64>, dim3(G), dim3(eig::NT), 3 * (size_t)n * sizeof(double2), s,
-                         A, n, j, V, taus, d, e, pin, pout);
+      hipLaunchKernelGGL(k_hetrd_step_wide, dim3(G), dim3(eig::NT), 2 * (size_t)n * sizeof(double2), s, A, n, j, V,
+                         taus, d, e, pin, pout);
   }
 }
 

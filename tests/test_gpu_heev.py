@@ -22,7 +22,7 @@ def _hermitian(n, seed, degenerate=False):
 
 
 @pytest.mark.parametrize("n,nev", [(2, 1), (3, 3), (17, 5), (160, 40), (333, 100), (640, 160), (1024, 256),
-                                   (1760, 440)])
+                                   (1025, 300), (1760, 440), (2048, 512)])
 def test_heev_matches_lapack(n, nev):
     from amaterasu_amd import ops
 
