@@ -1,7 +1,7 @@
-"""Exchange-correlation on the GPU in FP64 (LDA-PW, PBE, PBEsol).
+"""Exchange-correlation on the GPU in FP64 (LDA-PZ, LDA-PW, PBE, PBEsol).
 
 The pointwise forms are those of ``amaterasu/xc/lda.py`` and ``amaterasu/xc/gga.py``
-(Slater exchange, PW92 with and without the modified parametrisation, the PBE
+(Slater exchange, PZ81, PW92 with and without the modified parametrisation, the PBE
 enhancement factor and gradient correction written with log1p/expm1); the grid driver
 follows ``amaterasu/xc/driver.py:108-164``: an unpolarised density is evaluated as n/2
 in both spin channels, gradients and the divergence of the GGA flux are taken in
@@ -26,6 +26,9 @@ _PW92 = {"a": (0.031091, 0.015545, 0.016887), "alpha1": (0.21370, 0.20548, 0.111
          "beta1": (7.5957, 14.1189, 10.357), "beta2": (3.5876, 6.1977, 3.6231),
          "beta3": (1.6382, 3.3662, 0.88026), "beta4": (0.49294, 0.62517, 0.49671), "fz20": 1.709921}
 _PW92_MOD = {**_PW92, "a": (0.0310907, 0.01554535, 0.0168869), "fz20": 1.709920934161365617563962776245}
+#: Perdew-Zunger 1981 correlation, (unpolarised, polarised) (``lda.py:78-87``)
+_PZ81 = {"gamma": (-0.1423, -0.0843), "beta1": (1.0529, 1.3981), "beta2": (0.3334, 0.2611),
+         "a": (0.0311, 0.01555), "b": (-0.048, -0.0269), "c": (0.0020, 0.0007), "d": (-0.0116, -0.0048)}
 _GGA = {"PBE": {"kappa": 0.8040, "mu": 0.2195149727645171, "beta": 0.06672455060314922,
                 "gamma": 0.031090690869654895034},
         "PBESOL": {"kappa": 0.8040, "mu": 0.1234567901234567901234567901234567901235, "beta": 0.046,
@@ -72,8 +75,29 @@ def _pw92_local(rs, z, par):
     return eps, dg0 + z4 * fz * (dg1 - dg0 + dg2 / f20) - fz * dg2 / f20, (4.0 * z3 * fz + z4 * dfz) * br - dfz * g2 / f20
 
 
-def _lda(up, dn):
-    """(eps, v_up, v_dn): Slater exchange + PW92 correlation."""
+def _pz81_branch(rs, i):
+    """eps_c, d eps_c/d rs of one spin limit: Pade form for rs >= 1, high-density expansion
+    below (``lda.py:203-234``)."""
+    g, b1, b2 = _PZ81["gamma"][i], _PZ81["beta1"][i], _PZ81["beta2"][i]
+    a, b, c, d = _PZ81["a"][i], _PZ81["b"][i], _PZ81["c"][i], _PZ81["d"][i]
+    srs = torch.sqrt(rs)
+    den = 1.0 + b1 * srs + b2 * rs
+    lr = torch.log(torch.clamp(rs, min=1e-300))
+    dense = rs < 1.0
+    eps = torch.where(dense, a * lr + b + c * rs * lr + d * rs, g / den)
+    deps = torch.where(dense, a / rs + c * (lr + 1.0) + d, -g * (0.5 * b1 / srs + b2) / (den * den))
+    return eps, deps
+
+
+def _pz81_local(rs, z):
+    e0, d0 = _pz81_branch(rs, 0)
+    e1, d1 = _pz81_branch(rs, 1)
+    fz = _fz(z)
+    return e0 + fz * (e1 - e0), d0 + fz * (d1 - d0), _dfz(z) * (e1 - e0)
+
+
+def _lda(up, dn, pz81: bool = False):
+    """(eps, v_up, v_dn): Slater exchange + PW92 (or PZ81) correlation."""
     n, rs, z = _rs_zeta(up, dn)
     act = n > DENSITY_THRESHOLD
     su, sd = torch.clamp(up, min=0.0), torch.clamp(dn, min=0.0)
@@ -82,7 +106,7 @@ def _lda(up, dn):
                      torch.zeros_like(n))
     vxu = torch.where(act, -(4 / 3) * X_FACTOR_C * cu, torch.zeros_like(n))
     vxd = torch.where(act, -(4 / 3) * X_FACTOR_C * cd, torch.zeros_like(n))
-    ec, dr, dz = _pw92_local(rs, z, _PW92)
+    ec, dr, dz = _pz81_local(rs, z) if pz81 else _pw92_local(rs, z, _PW92)
     common = ec - (rs / 3.0) * dr
     zero = torch.zeros_like(n)
     ec = torch.where(act, ec, zero)
@@ -200,14 +224,14 @@ class XCFunctional:
 
     def __init__(self, name: str):
         key = name.strip().upper().replace("_", "-")
-        if key not in ("LDA-PW", "PBE", "PBESOL"):
-            raise ValueError(f"unknown functional {name!r}; available: LDA-PW, PBE, PBEsol")
+        if key not in ("LDA-PZ", "LDA-PW", "PBE", "PBESOL"):
+            raise ValueError(f"unknown functional {name!r}; available: LDA-PZ, LDA-PW, PBE, PBEsol")
         self.key = key
         self.name = {"PBESOL": "PBEsol"}.get(key, key)
 
     @property
     def is_gga(self) -> bool:
-        return self.key != "LDA-PW"
+        return self.key not in ("LDA-PZ", "LDA-PW")
 
     def evaluate_on_grid(self, density: torch.Tensor, gops: GridOps):
         """(energy, potential (nspin, *grid)) for density (nspin, *grid) FP64 on the device."""
@@ -223,7 +247,7 @@ class XCFunctional:
         up, dn = (d[0] / 2, d[0] / 2) if nspin == 1 else (d[0], d[1])
         dv = gops.volume / gops.size
         if not self.is_gga:
-            eps, vu, vd = _lda(up, dn)
+            eps, vu, vd = _lda(up, dn, pz81=self.key == "LDA-PZ")
             energy = float((eps * (up + dn)).sum().item()) * dv
             pot = vu[None] if nspin == 1 else torch.stack([vu, vd])
             return energy, pot, None, None

@@ -90,6 +90,35 @@ def _pw92_local(rs, z, par):
     return eps, d_drs, d_dz
 
 
+#: Perdew-Zunger 1981 (PRB 23, 5048) correlation parameters, (unpolarised, polarised):
+#: the published table (``lda.py:78-87``).
+PZ81 = {"gamma": (-0.1423, -0.0843), "beta1": (1.0529, 1.3981), "beta2": (0.3334, 0.2611),
+        "a": (0.0311, 0.01555), "b": (-0.048, -0.0269), "c": (0.0020, 0.0007), "d": (-0.0116, -0.0048)}
+
+
+def pz81_branch(rs, i):
+    """eps_c and d eps_c / d rs of one spin limit: the Pade form for rs >= 1, the
+    high-density expansion below (``lda.py:203-234``)."""
+    g, b1, b2 = PZ81["gamma"][i], PZ81["beta1"][i], PZ81["beta2"][i]
+    a, b, c, d = PZ81["a"][i], PZ81["b"][i], PZ81["c"][i], PZ81["d"][i]
+    srs = np.sqrt(rs)
+    den = 1.0 + b1 * srs + b2 * rs
+    low, dlow = g / den, -g * (0.5 * b1 / srs + b2) / den**2
+    lr = np.log(np.maximum(rs, 1e-300))
+    high, dhigh = a * lr + b + c * rs * lr + d * rs, a / rs + c * (lr + 1.0) + d
+    dense = rs < 1.0
+    return np.where(dense, high, low), np.where(dense, dhigh, dlow)
+
+
+def correlation_pz81(rho):
+    """``lda.py:203-244``: the two limits interpolated with f(zeta)."""
+    rho = np.atleast_2d(np.asarray(rho, dtype=float))
+    n, rs, z = rs_and_zeta(rho)
+    (e0, d0), (e1, d1) = pz81_branch(rs, 0), pz81_branch(rs, 1)
+    fz, dfz = f_zeta(z), f_zeta_derivative(z)
+    return _assemble(e0 + fz * (e1 - e0), d0 + fz * (d1 - d0), dfz * (e1 - e0), rs, z, n > DENSITY_THRESHOLD)
+
+
 def correlation_pw92(rho, *, modified=False):
     """``lda.py:169-199``."""
     rho = np.atleast_2d(np.asarray(rho, dtype=float))
@@ -168,14 +197,15 @@ def gga_correlation(rho, sigma, par):
 
 
 class XCFunctional:
-    """``driver.py:67-164`` for LDA-PW, PBE and PBEsol."""
+    """``driver.py:67-164`` for LDA-PZ, LDA-PW, PBE and PBEsol."""
 
     def __init__(self, name: str):
         key = name.strip().upper().replace("_", "-")
-        table = {"LDA-PW": "lda", "PBE": "gga", "PBESOL": "gga"}
+        table = {"LDA-PZ": "lda", "LDA-PW": "lda", "PBE": "gga", "PBESOL": "gga"}
         if key not in table:
             raise ValueError(f"unknown functional {name!r}")
         self.kind = table[key]
+        self.key = key
         self.name = {"PBESOL": "PBEsol"}.get(key, key)
         self.params = PBESOL if key == "PBESOL" else PBE
 
@@ -187,7 +217,7 @@ class XCFunctional:
         rho = np.atleast_2d(np.asarray(rho, dtype=float))
         if self.kind == "lda":
             ex, vx = lda_exchange(rho)
-            ec, vc = correlation_pw92(rho)
+            ec, vc = correlation_pz81(rho) if self.key == "LDA-PZ" else correlation_pw92(rho)
             return ex + ec, vx + vc
         sigma = np.atleast_2d(np.asarray(sigma, dtype=float))
         ex, vx, sx = gga_exchange(rho, sigma, self.params)

@@ -17,7 +17,7 @@ def test_lda_exchange_of_the_uniform_gas():
     assert v[:, 0] == pytest.approx((4 / 3) * eps, rel=1e-14)
 
 
-@pytest.mark.parametrize("name", ["LDA-PW", "PBE", "PBEsol"])
+@pytest.mark.parametrize("name", ["LDA-PZ", "LDA-PW", "PBE", "PBEsol"])
 def test_potential_is_the_derivative_of_the_energy(name):
     """v_s = d(n eps)/dn_s and vsigma = d(n eps)/dsigma by central differences: pins every
     derivative chain in the restatement independently of the energy formula."""
@@ -46,6 +46,27 @@ def test_potential_is_the_derivative_of_the_energy(name):
             sm[:, c] -= h
             fd = (energy(rho, sp) - energy(rho, sm)) / (2 * h)
             assert out[2][:, c] == pytest.approx(fd, rel=1e-5, abs=1e-8)
+
+
+def test_pz81_against_pw92_and_its_own_branches():
+    """PZ81 (lda.py:203-244): its two branches meet at rs = 1 (the published coefficients
+    make the value continuous to 3.2e-5 Ha); it agrees with PW92 -- an independent fit of the
+    same Ceperley-Alder data -- to 1.5 mHa over rs 0.5..6 in both spin limits; and for
+    large rs it approaches the Pade limit gamma / (beta2 rs), with the relative correction
+    beta1 / (beta2 sqrt(rs))."""
+    rs = np.linspace(0.5, 6.0, 23)
+    n = 3.0 / (4.0 * np.pi * rs**3)
+    for pol in (0, 1):
+        rho = np.stack([n, np.zeros_like(n)], axis=1) if pol else np.stack([n / 2, n / 2], axis=1)
+        pz, _ = xc.correlation_pz81(rho)
+        pw, _ = xc.correlation_pw92(rho)
+        assert np.abs(pz - pw).max() < 1.5e-3
+        lo, _ = xc.pz81_branch(np.array([1.0 + 1e-12]), pol)
+        hi, _ = xc.pz81_branch(np.array([1.0 - 1e-12]), pol)
+        assert abs(lo[0] - hi[0]) < 4e-5
+    big = np.array([1e8])
+    e, _ = xc.pz81_branch(big, 0)
+    assert e[0] == pytest.approx(xc.PZ81["gamma"][0] / (xc.PZ81["beta2"][0] * big[0]), rel=5e-4)
 
 
 def test_pbe_reduces_to_lda_without_gradient():
