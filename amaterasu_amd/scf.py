@@ -19,6 +19,7 @@ this package's :class:`~amaterasu_amd.system.DeviceSystem` both provide it.
 
 from __future__ import annotations
 
+import math
 import time
 from dataclasses import dataclass, field
 
@@ -45,6 +46,7 @@ class SCFSettings:
     tol_density: float = 1e-7
     scheme: str = "fixed"
     smearing: float = 0.0  # Hartree
+    smearing_order: int = 1  # Methfessel-Paxton order
     mixing: str = "pulay"
     beta: float = 0.4
     history: int = 8
@@ -169,9 +171,24 @@ class GridSide:
         return out
 
 
+def _hermite_gauss(x: torch.Tensor, mmax: int) -> list:
+    """H_m(x) e^{-x^2} for m = 0..mmax (H_{m+1} = 2x H_m - 2m H_{m-1})."""
+    e = torch.exp(-torch.clamp(x * x, max=200.0))
+    h = [e, 2.0 * x * e]
+    for m in range(1, mmax):
+        h.append(2.0 * x * h[m] - 2.0 * m * h[m - 1])
+    return h
+
+
+def _mp_coefficient(n: int) -> float:
+    """Methfessel-Paxton A_n = (-1)^n / (n! 4^n sqrt(pi)) (PRB 40, 3616)."""
+    return (-1.0) ** n / (math.factorial(n) * 4.0**n * np.sqrt(np.pi))
+
+
 def occupations(eig: torch.Tensor, weights: torch.Tensor, nelectrons: float, scheme: str, smearing: float,
-                nspin: int, tolerance: float = 1e-12):
-    """``occupations.py:134-244`` (fixed, Fermi-Dirac, Gaussian, Marzari-Vanderbilt) in FP64 on the device.
+                nspin: int, tolerance: float = 1e-12, order: int = 1):
+    """``occupations.py:134-244`` (fixed, Fermi-Dirac, Gaussian, Methfessel-Paxton of ``order``,
+    Marzari-Vanderbilt) in FP64 on the device.
     Returns (occupations, fermi level, entropy term, gap or None)."""
     g = 2.0 / nspin
     w = weights.to(eig)
@@ -200,6 +217,9 @@ def occupations(eig: torch.Tensor, weights: torch.Tensor, nelectrons: float, sch
             return 0.5 * torch.erfc(sh) + torch.exp(-torch.clamp(sh * sh, max=200.0)) / np.sqrt(2.0 * np.pi)
         if scheme == "gaussian":
             return 0.5 * torch.erfc(x)
+        if scheme == "methfessel-paxton":  # occupations.py:47-68,84-89
+            h = _hermite_gauss(x, 2 * order)
+            return 0.5 * torch.erfc(x) + sum(_mp_coefficient(n) * h[2 * n - 1] for n in range(1, order + 1))
         raise ValueError(f"unknown smearing scheme {scheme!r}")
 
     def entfun(x):
@@ -209,6 +229,10 @@ def occupations(eig: torch.Tensor, weights: torch.Tensor, nelectrons: float, sch
         if scheme == "marzari-vanderbilt":
             sh = x + 1.0 / np.sqrt(2.0)
             return sh * torch.exp(-torch.clamp(sh * sh, max=200.0)) / np.sqrt(2.0 * np.pi)
+        if scheme == "methfessel-paxton":  # S' = x f' (occupations.py:106-111)
+            h = _hermite_gauss(x, 2 * order)
+            return 0.5 * h[0] / np.sqrt(np.pi) + sum(_mp_coefficient(n) * (0.5 * h[2 * n] + 2 * n * h[2 * n - 2])
+                                                     for n in range(1, order + 1))
         return 0.5 * torch.exp(-torch.clamp(x * x, max=200.0)) / np.sqrt(np.pi)
 
     def electrons(mu):
@@ -402,7 +426,7 @@ def run_scf(system, settings: SCFSettings, *, density=None, wavefunctions=None, 
                 wf[sp, ik, :, :n] = r.coefficients.to(cdt)
                 dav_steps += r.iterations
         occ, fermi, ent, gap = occupations(eig, weights, system.nelectrons, settings.scheme, settings.smearing,
-                                           nspin)
+                                           nspin, order=settings.smearing_order)
         # output density: one batched launch over all k per spin
         dout = torch.zeros_like(rho)
         f_all = (2.0 / nspin) * weights[None, :, None] * occ

@@ -14,6 +14,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
+import math
+
 import numpy as np
 from scipy.special import erfc
 
@@ -41,9 +43,26 @@ def hartree(grid, sphere, density):
 _MAXARG = 200.0
 
 
-def occupation_function(x, scheme):
-    """``occupations.py:71-89`` (no Methfessel-Paxton)."""
+def _hermite_gauss(x, mmax):
+    """H_m(x) e^{-x^2} for m = 0..mmax, by H_{m+1} = 2x H_m - 2m H_{m-1}."""
+    e = np.exp(-np.minimum(_MAXARG, x**2))
+    h = [e, 2.0 * x * e]
+    for m in range(1, mmax):
+        h.append(2.0 * x * h[m] - 2.0 * m * h[m - 1])
+    return h
+
+
+def _mp_coefficient(n):
+    """Methfessel-Paxton (PRB 40, 3616) A_n = (-1)^n / (n! 4^n sqrt(pi))."""
+    return (-1.0) ** n / (math.factorial(n) * 4.0**n * np.sqrt(np.pi))
+
+
+def occupation_function(x, scheme, order=1):
+    """``occupations.py:71-89``."""
     x = np.asarray(x, dtype=float)
+    if scheme == "methfessel-paxton":  # occupations.py:47-68,84-89
+        h = _hermite_gauss(x, 2 * order)
+        return 0.5 * erfc(x) + sum(_mp_coefficient(n) * h[2 * n - 1] for n in range(1, order + 1))
     if scheme == "fermi-dirac":
         return 1.0 / (1.0 + np.exp(np.clip(x, -_MAXARG, _MAXARG)))
     if scheme == "marzari-vanderbilt":
@@ -54,9 +73,14 @@ def occupation_function(x, scheme):
     raise ValueError(f"unknown smearing scheme {scheme!r}")
 
 
-def entropy_function(x, scheme):
-    """``occupations.py:92-111`` (no Methfessel-Paxton)."""
+def entropy_function(x, scheme, order=1):
+    """``occupations.py:92-111``; for Methfessel-Paxton S = e^{-x^2}/(2 sqrt pi) +
+    sum_n A_n (H_{2n}/2 + 2n H_{2n-2}) e^{-x^2}, the function with S' = x f'."""
     x = np.asarray(x, dtype=float)
+    if scheme == "methfessel-paxton":  # occupations.py:106-111
+        h = _hermite_gauss(x, 2 * order)
+        return 0.5 * h[0] / np.sqrt(np.pi) + sum(_mp_coefficient(n) * (0.5 * h[2 * n] + 2 * n * h[2 * n - 2])
+                                                 for n in range(1, order + 1))
     if scheme == "fermi-dirac":
         f = np.clip(occupation_function(x, scheme), 1e-300, 1.0 - 1e-16)
         return -(f * np.log(f) + (1.0 - f) * np.log1p(-f))
@@ -76,7 +100,8 @@ class OccupationResult:
     gap: float | None
 
 
-def compute_occupations(eig, weights, nelectrons, *, scheme="gaussian", smearing=0.0, nspin=1, tolerance=1e-12):
+def compute_occupations(eig, weights, nelectrons, *, scheme="gaussian", smearing=0.0, order=1, nspin=1,
+                        tolerance=1e-12):
     """``occupations.py:157-244``: zero-temperature filling or a 200-step bisection on mu."""
     eig = np.asarray(eig, dtype=float)
     w = np.asarray(weights, dtype=float)
@@ -102,7 +127,7 @@ def compute_occupations(eig, weights, nelectrons, *, scheme="gaussian", smearing
     hi = float(eig.max()) + 20.0 * smearing + 1.0
 
     def electrons(mu):
-        return g * float(np.einsum("k,skn->", w, occupation_function((eig - mu) / smearing, scheme)))
+        return g * float(np.einsum("k,skn->", w, occupation_function((eig - mu) / smearing, scheme, order)))
 
     if electrons(hi) < nelectrons - 1e-9:
         raise ValueError("not enough bands for the electrons")
@@ -116,8 +141,8 @@ def compute_occupations(eig, weights, nelectrons, *, scheme="gaussian", smearing
             break
     mu = 0.5 * (lo + hi)
     x = (eig - mu) / smearing
-    occ = occupation_function(x, scheme)
-    ent = smearing * g * float(np.einsum("k,skn->", w, entropy_function(x, scheme)))
+    occ = occupation_function(x, scheme, order)
+    ent = smearing * g * float(np.einsum("k,skn->", w, entropy_function(x, scheme, order)))
     gap = None
     if float(np.max(np.minimum(occ, 1.0 - occ))) < 1e-6:
         filled, empty = eig[occ > 0.5], eig[occ <= 0.5]
@@ -205,6 +230,7 @@ class SCFSettings:
     tol_density: float = 1e-7
     scheme: str = "fixed"
     smearing: float = 0.0  # Ha
+    smearing_order: int = 1  # Methfessel-Paxton order
     mixing: str = "pulay"
     beta: float = 0.4
     history: int = 8
@@ -288,7 +314,7 @@ def run_scf(system, cfg: SCFSettings, *, wavefunctions=None, density=None, log=N
                 eig[sp, ik] = r.eigenvalues
                 wf[sp][ik] = r.coefficients
         occ = compute_occupations(eig, system.kpoints.weights, system.nelectrons, scheme=cfg.scheme,
-                                  smearing=cfg.smearing, nspin=system.nspin)
+                                  smearing=cfg.smearing, order=cfg.smearing_order, nspin=system.nspin)
         dout = compute_density(system, wf, occ.occupations)
         eh, _ = hartree(grid, system.density_sphere, dout.sum(axis=0))
         xd = dout if system.core_density is None else dout + system.core_density[None] / system.nspin

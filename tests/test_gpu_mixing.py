@@ -39,10 +39,12 @@ def test_pulay_kerker_mixer_matches_the_oracle(si2_system, q0, linear, history):
         x = want
 
 
-@pytest.mark.parametrize("scheme,smearing,nspin", [("fixed", 0.0, 1), ("fermi-dirac", 0.01, 1),
-                                                   ("gaussian", 0.02, 1), ("marzari-vanderbilt", 0.02, 1),
-                                                   ("fermi-dirac", 0.005, 2)])
-def test_occupations_match_the_oracle(scheme, smearing, nspin):
+@pytest.mark.parametrize("scheme,smearing,nspin,order", [("fixed", 0.0, 1, 1), ("fermi-dirac", 0.01, 1, 1),
+                                                         ("gaussian", 0.02, 1, 1), ("marzari-vanderbilt", 0.02, 1, 1),
+                                                         ("methfessel-paxton", 0.02, 1, 1),
+                                                         ("methfessel-paxton", 0.02, 1, 2),
+                                                         ("fermi-dirac", 0.005, 2, 1)])
+def test_occupations_match_the_oracle(scheme, smearing, nspin, order):
     import torch
 
     from amaterasu_amd.scf import occupations
@@ -53,8 +55,8 @@ def test_occupations_match_the_oracle(scheme, smearing, nspin):
     w = np.array([0.1, 0.2, 0.3, 0.4])
     nel = 13.0 if nspin == 1 else 13.5
     occ, mu, ent, gap = occupations(torch.from_numpy(eig).cuda(), torch.from_numpy(w).cuda(), nel, scheme,
-                                    smearing, nspin)
-    ref = compute_occupations(eig, w, nel, scheme=scheme, smearing=smearing, nspin=nspin)
+                                    smearing, nspin, order=order)
+    ref = compute_occupations(eig, w, nel, scheme=scheme, smearing=smearing, order=order, nspin=nspin)
     assert np.abs(occ.cpu().numpy() - ref.occupations).max() < 1e-9
     assert mu == pytest.approx(ref.fermi_level, abs=1e-11)
     assert ent == pytest.approx(ref.entropy_term, abs=1e-12)

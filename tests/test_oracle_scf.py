@@ -128,7 +128,7 @@ def test_hartree_of_gaussians_is_the_ewald_reciprocal_sum():
     assert 0.5 * grid.integrate(v * dens) == pytest.approx(e, rel=1e-9)
 
 
-@pytest.mark.parametrize("scheme", ["fermi-dirac", "gaussian", "marzari-vanderbilt"])
+@pytest.mark.parametrize("scheme", ["fermi-dirac", "gaussian", "marzari-vanderbilt", "methfessel-paxton"])
 def test_occupations_hold_the_electron_count(scheme):
     from oracle.scf import compute_occupations
 
@@ -139,6 +139,29 @@ def test_occupations_hold_the_electron_count(scheme):
     assert 2.0 * np.einsum("k,skn->", w, r.occupations) == pytest.approx(9.3, abs=1e-9)
     fixed = compute_occupations(eig, w, 8.0, scheme="fixed")
     assert 2.0 * np.einsum("k,skn->", w, fixed.occupations) == pytest.approx(8.0, abs=1e-12)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_methfessel_paxton_moments_and_entropy(order):
+    """Methfessel-Paxton of order N (PRB 40, 3616; occupations.py:47-111): the smeared delta
+    -f' integrates to one and its moments x^2 .. x^{2N} vanish (x^{2N+2} does not); N = 0
+    would be the Gaussian; the generalised entropy obeys S' = x f' and vanishes far from
+    the Fermi level. A wrong sign of A_n or a shifted Hermite index breaks the moments."""
+    from oracle.scf import entropy_function, occupation_function
+
+    x = np.linspace(-7.0, 7.0, 28001)
+    h = x[1] - x[0]
+    f = occupation_function(x, "methfessel-paxton", order)
+    ent = entropy_function(x, "methfessel-paxton", order)
+    df = np.gradient(f, h)
+    assert np.trapezoid(-df, x) == pytest.approx(1.0, abs=1e-9)
+    for k in range(1, order + 1):
+        assert abs(np.trapezoid(-df * x ** (2 * k), x)) < 1e-6
+    assert abs(np.trapezoid(-df * x ** (2 * order + 2), x)) > 0.1
+    assert np.abs(np.gradient(ent, h) - x * df).max() < 1e-6
+    assert abs(ent[0]) < 1e-12 and abs(ent[-1]) < 1e-12
+    gauss = occupation_function(x, "gaussian")
+    assert np.abs(occupation_function(x, "methfessel-paxton", 0) - gauss).max() < 1e-15
 
 
 def test_pulay_reaches_the_fixed_point_of_a_linear_map():
