@@ -190,6 +190,11 @@ def occupations(eig: torch.Tensor, weights: torch.Tensor, nelectrons: float, sch
     """``occupations.py:134-244`` (fixed, Fermi-Dirac, Gaussian, Methfessel-Paxton of ``order``,
     Marzari-Vanderbilt) in FP64 on the device.
     Returns (occupations, fermi level, entropy term, gap or None)."""
+    if eig.dim() != 3:
+        raise ValueError(f"eigenvalues must be (nspin, nk, nbands), got {tuple(eig.shape)}")
+    if scheme == "tetrahedron":  # occupations.py:192-196
+        raise NotImplementedError("the linear tetrahedron method is not implemented; use occupations.scheme: "
+                                  "gaussian, methfessel-paxton or marzari-vanderbilt")
     g = 2.0 / nspin
     w = weights.to(eig)
     if scheme == "fixed" or smearing <= 0.0:

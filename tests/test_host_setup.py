@@ -96,3 +96,18 @@ def test_smooth_potential_is_seeded_and_normalised():
     assert np.array_equal(v, smooth_potential((12, 12, 12), seed=3))
     assert np.sqrt(np.mean(v**2)) == pytest.approx(1.0)
     assert abs(v.mean()) < 1e-12
+
+
+def test_occupation_errors_follow_the_reference():
+    """occupations.py:187-196: a non-(nspin, nk, nbands) array is a ValueError, the
+    tetrahedron scheme a NotImplementedError (host tensors; no GPU needed)."""
+    import torch
+
+    from amaterasu_amd.scf import occupations
+
+    eig = torch.zeros((1, 2, 4), dtype=torch.float64)
+    w = torch.tensor([0.5, 0.5], dtype=torch.float64)
+    with pytest.raises(ValueError):
+        occupations(eig[0], w, 4.0, "gaussian", 0.01, 1)
+    with pytest.raises(NotImplementedError):
+        occupations(eig, w, 4.0, "tetrahedron", 0.01, 1)
