@@ -132,8 +132,22 @@ class GridSide:
         return e, self.g.to_real(c * self.coulomb).real.contiguous()
 
     def kerker(self, q0: float):
+        """``mixing.py:49-106``: G^2/(G^2+q0^2) on the charge residual over every grid
+        frequency (G = 0 removed); with two spin channels the moment passes unfiltered."""
         f = torch.where(self.g.g2 > 1e-12, self.g.g2 / (self.g.g2 + q0 * q0), torch.zeros_like(self.g.g2))
-        return lambda r: self.g.to_real(self.g.to_reciprocal(r[0]) * f).real[None] if q0 > 0 else r
+
+        def filt(x):
+            return self.g.to_real(self.g.to_reciprocal(x) * f).real
+
+        def apply(r):
+            if q0 <= 0:
+                return r
+            if r.shape[0] == 1:
+                return filt(r[0])[None]
+            charge, moment = filt(r[0] + r[1]), r[0] - r[1]
+            return torch.stack([0.5 * (charge + moment), 0.5 * (charge - moment)])
+
+        return apply
 
     def _symmetry_maps(self, system):
         """Per operation: the grid index of R^T G for each density-sphere G, and exp(-2 pi i G.f)."""
@@ -328,6 +342,56 @@ class PulayMixer:
         return xin + self.beta * self.precondition(xr)
 
 
+class BroydenMixer:
+    """Johnson's modified Broyden (PRB 38, 12807; ``mixing.py:217-268``) with the history in
+    device memory: secant pairs of successive residual and input differences, normalised by
+    the residual change; ``w0`` regularises the normal equations."""
+
+    def __init__(self, beta=0.4, history=8, precondition=None, w0=0.01, weight=1.0):
+        self.beta, self.history, self.w0, self.weight = beta, history, w0, weight
+        self.precondition = precondition or (lambda r: r)
+        self.inputs, self.residuals = [], []
+
+    def __call__(self, din: torch.Tensor, dout: torch.Tensor) -> torch.Tensor:
+        r = dout - din
+        self.inputs.append(din.clone())
+        self.residuals.append(r)
+        if len(self.inputs) > self.history:
+            del self.inputs[0]
+            del self.residuals[0]
+        step = self.beta * self.precondition(r)
+        if len(self.residuals) < 2:
+            return din + step
+        df, dn = [], []
+        for i in range(len(self.residuals) - 1):
+            diff = self.residuals[i + 1] - self.residuals[i]
+            nrm = float(torch.linalg.vector_norm(diff).item())
+            if nrm < 1e-30:
+                continue
+            df.append(diff.reshape(-1) / nrm)
+            dn.append((self.inputs[i + 1] - self.inputs[i] + self.beta * self.precondition(diff)).reshape(-1) / nrm)
+        if not df:
+            return din + step
+        dfm, dnm = torch.stack(df), torch.stack(dn)
+        m = self.weight**2 * (dfm @ dfm.T) + self.w0**2 * torch.eye(len(df), dtype=dfm.dtype, device=dfm.device)
+        gamma = torch.linalg.solve(m, self.weight * (dfm @ r.reshape(-1))) * self.weight
+        return din + step - (gamma @ dnm).reshape(din.shape)
+
+
+def make_mixer(name: str, beta: float = 0.4, history: int = 8, precondition=None):
+    """``mixing.py:263-286``: linear, anderson (Pulay over at most two iterations), pulay,
+    broyden; an unknown name is a ValueError."""
+    if name == "linear":
+        return PulayMixer(beta, history, precondition, linear=True)
+    if name == "anderson":
+        return PulayMixer(beta, min(history, 2), precondition)
+    if name == "pulay":
+        return PulayMixer(beta, history, precondition)
+    if name == "broyden":
+        return BroydenMixer(beta, history, precondition)
+    raise ValueError(f"unknown mixing algorithm {name!r}; available: linear, anderson, pulay, broyden")
+
+
 def _as_device(x, dtype=torch.float64):
     if x is None:
         return None
@@ -391,9 +455,8 @@ def run_scf(system, settings: SCFSettings, *, density=None, wavefunctions=None, 
     wf = wavefunctions if wavefunctions is not None else initial_wavefunctions(system, plan, cdt)
     wf = wf.to(cdt)
     weights = torch.as_tensor(np.asarray(system.kpoints.weights, dtype=float), device="cuda")
-    mixer = PulayMixer(settings.beta, settings.history,
-                       side.kerker(settings.kerker_q0) if settings.kerker_q0 > 0 else None,
-                       linear=settings.mixing == "linear")
+    mixer = make_mixer(settings.mixing, settings.beta, settings.history,
+                       side.kerker(settings.kerker_q0) if settings.kerker_q0 > 0 else None)
     ethr = settings.tol_energy * natoms * EV_IN_HARTREE
     dv = system.grid.cell_volume_per_point
     hist, prev, resid, conv = [], np.inf, np.inf, False
@@ -491,4 +554,5 @@ def run_scf(system, settings: SCFSettings, *, density=None, wavefunctions=None, 
                      hist, count, time.perf_counter() - t_start, dav_steps, magnetization)
 
 
-__all__ = ["Energies", "GridSide", "PulayMixer", "SCFResult", "SCFSettings", "occupations", "run_scf"]
+__all__ = ["BroydenMixer", "Energies", "GridSide", "PulayMixer", "SCFResult", "SCFSettings", "make_mixer",
+           "occupations", "run_scf"]

@@ -162,12 +162,17 @@ class Kerker:
             self.factor = np.where(g2 > 1e-12, g2 / (g2 + q0**2), 0.0)
         self.grid, self.q0 = grid, q0
 
+    def _filter(self, f):
+        return self.grid.to_real(self.grid.to_reciprocal(f.astype(complex)) * self.factor).real
+
     def __call__(self, r):
+        """``mixing.py:97-106``: the charge residual is filtered, the moment passes through."""
         if self.q0 <= 0:
             return r
-        if r.shape[0] != 1:
-            raise NotImplementedError("the oracle restates the spin-unpolarised mixer")
-        return self.grid.to_real(self.grid.to_reciprocal(r[0].astype(complex)) * self.factor).real[None]
+        if r.shape[0] == 1:
+            return self._filter(r[0])[None]
+        charge, moment = self._filter(r[0] + r[1]), r[0] - r[1]
+        return np.stack([0.5 * (charge + moment), 0.5 * (charge - moment)])
 
 
 class PulayMixer:
@@ -218,6 +223,50 @@ class PulayMixer:
         xin = np.einsum("i,i...->...", c, np.array(self.inputs))
         xr = np.einsum("i,i...->...", c, np.array(self.residuals))
         return xin + self.beta * self.precondition(xr)
+
+
+class BroydenMixer(PulayMixer):
+    """Johnson's modified Broyden (PRB 38, 12807; ``mixing.py:217-268``): secant pairs of
+    successive residual and input differences, normalised by the residual change, weighted
+    by ``weight`` against the regulariser ``w0``."""
+
+    def __init__(self, beta=0.4, history=8, precondition=None, w0=0.01, weight=1.0):
+        super().__init__(beta, history, precondition)
+        self.w0, self.weight = w0, weight
+
+    def __call__(self, din, dout):
+        r = self._record(din, dout)
+        step = self.beta * self.precondition(r)
+        if len(self.residuals) < 2:
+            return din + step
+        df, dn = [], []
+        for i in range(len(self.residuals) - 1):
+            diff = self.residuals[i + 1] - self.residuals[i]
+            nrm = float(np.linalg.norm(diff))
+            if nrm < 1e-30:
+                continue
+            df.append(diff.ravel() / nrm)
+            dn.append((self.inputs[i + 1] - self.inputs[i] + self.beta * self.precondition(diff)).ravel() / nrm)
+        if not df:
+            return din + step
+        df, dn = np.array(df), np.array(dn)
+        m = self.weight**2 * (df @ df.T) + self.w0**2 * np.eye(len(df))
+        gamma = np.linalg.solve(m, self.weight * (df @ r.ravel())) * self.weight
+        return din + step - (gamma @ dn).reshape(din.shape)
+
+
+def make_mixer(name, beta=0.4, history=8, precondition=None):
+    """``mixing.py:263-286``: linear, anderson (Pulay over at most two iterations), pulay,
+    broyden."""
+    if name == "linear":
+        return PulayMixer(beta, history, precondition, linear=True)
+    if name == "anderson":
+        return PulayMixer(beta, min(history, 2), precondition)
+    if name == "pulay":
+        return PulayMixer(beta, history, precondition)
+    if name == "broyden":
+        return BroydenMixer(beta, history, precondition)
+    raise ValueError(f"unknown mixing algorithm {name!r}")
 
 
 # ------------------------------------------------------------------------------ SCF
@@ -294,8 +343,7 @@ def run_scf(system, cfg: SCFSettings, *, wavefunctions=None, density=None, log=N
     natoms = len(system.symbols)
     density = system.initial_density() if density is None else np.array(density, dtype=float)
     wf = wavefunctions or initial_wavefunctions(system)
-    mixer = PulayMixer(cfg.beta, cfg.history, Kerker(grid, cfg.kerker_q0) if cfg.kerker_q0 > 0 else None,
-                       linear=cfg.mixing == "linear")
+    mixer = make_mixer(cfg.mixing, cfg.beta, cfg.history, Kerker(grid, cfg.kerker_q0) if cfg.kerker_q0 > 0 else None)
     ethr = cfg.tol_energy * natoms * EV_IN_HARTREE
     hist, prev, resid, conv = [], np.inf, np.inf, False
     energies, occ = Energies(ewald=system.ewald), None

@@ -39,6 +39,40 @@ def test_pulay_kerker_mixer_matches_the_oracle(si2_system, q0, linear, history):
         x = want
 
 
+@pytest.mark.parametrize("name", ["broyden", "anderson", "linear"])
+@pytest.mark.parametrize("nspin", [1, 2])
+def test_mixers_match_the_oracle(si2_system, name, nspin):
+    """Broyden (Johnson), Anderson and linear mixing with the Kerker preconditioner, one and
+    two spin channels (the charge filtered, the moment passed through), step by step
+    against the oracle on the same noisy nonlinear map."""
+    import torch
+
+    from amaterasu_amd.scf import GridSide, make_mixer
+    from oracle.scf import Kerker
+    from oracle.scf import make_mixer as omake
+
+    shape = (nspin, *si2_system.grid.shape)
+    rng = np.random.default_rng(31 + nspin)
+    dm = make_mixer(name, 0.4, 5, GridSide(si2_system).kerker(1.5))
+    om = omake(name, 0.4, 5, Kerker(si2_system.grid, 1.5))
+    target = 0.05 + np.abs(rng.normal(size=shape))
+    x = np.abs(rng.normal(size=shape))
+    for _ in range(8):
+        out = target + 0.5 * np.tanh(x - target) + 0.05 * rng.normal(size=shape)
+        got = dm(torch.from_numpy(x).cuda(), torch.from_numpy(out).cuda()).cpu().numpy()
+        want = om(x, out)
+        assert got.shape == want.shape
+        assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
+        x = want
+
+
+def test_unknown_mixer_is_refused():
+    from amaterasu_amd.scf import make_mixer
+
+    with pytest.raises(ValueError):
+        make_mixer("simplex")
+
+
 @pytest.mark.parametrize("scheme,smearing,nspin,order", [("fixed", 0.0, 1, 1), ("fermi-dirac", 0.01, 1, 1),
                                                          ("gaussian", 0.02, 1, 1), ("marzari-vanderbilt", 0.02, 1, 1),
                                                          ("methfessel-paxton", 0.02, 1, 1),

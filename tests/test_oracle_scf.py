@@ -197,3 +197,48 @@ def test_scf_invariants(si2_system):
     assert abs(r.history[-1][2]) * HARTREE_IN_EV < 1e-9
     energies = [h[0] for h in r.history]
     assert energies[-1] == pytest.approx(min(energies), abs=1e-12)
+
+
+def test_spin_kerker_filters_the_charge_only(si2_system):
+    """mixing.py:49-106 for two spin channels: equal channels (no moment) are filtered as the
+    total charge, opposite channels (pure moment) pass through unchanged, and the G = 0
+    component of the charge is removed."""
+    from oracle.scf import Kerker
+
+    grid = si2_system.grid
+    rng = np.random.default_rng(21)
+    r = rng.normal(size=(1, *grid.shape))
+    k = Kerker(grid, 1.5)
+    same = k(np.concatenate([r, r]))
+    one = k(2 * r)[0] / 2
+    assert np.abs(same[0] - one).max() < 1e-13 and np.abs(same[1] - one).max() < 1e-13
+    opposite = np.concatenate([r, -r])
+    assert np.abs(k(opposite) - opposite).max() < 1e-13
+    assert abs(k(r)[0].mean()) < 1e-13
+
+
+def test_broyden_and_anderson_reach_the_fixed_point_of_a_linear_map():
+    """Johnson's Broyden (mixing.py:217-268) and Anderson (Pulay over two iterations) on
+    x -> b + A x with a contraction A: both converge to (I - A)^{-1} b, Broyden in fewer
+    steps than linear mixing."""
+    from oracle.scf import make_mixer
+
+    rng = np.random.default_rng(6)
+    n = 40
+    q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    a = (q * rng.uniform(-0.8, 0.8, size=n)) @ q.T
+    b = rng.normal(size=n)
+    fixed = np.linalg.solve(np.eye(n) - a, b)
+
+    def run(name, steps):
+        m = make_mixer(name, beta=0.4, history=8)
+        x = np.zeros(n)
+        for _ in range(steps):
+            x = m(x, b + a @ x)
+        return np.abs(x - fixed).max()
+
+    assert run("broyden", 40) < 1e-8
+    assert run("anderson", 80) < 1e-5
+    assert run("broyden", 25) < run("linear", 25)
+    with pytest.raises(ValueError):
+        make_mixer("simplex")
