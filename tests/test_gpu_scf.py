@@ -92,3 +92,32 @@ def test_smeared_scf_matches_oracle(si2_system):
     assert abs(got.energies.free_energy - ref.energies.free_energy) < 1e-8
     assert abs(got.fermi_level - ref.fermi_level) < 1e-7
     assert np.abs(got.occupations - ref.occupations).max() < 1e-6
+
+
+def test_spin_polarised_scf_matches_oracle(si_pseudo, si2_system):
+    """Two spin channels from a polarised start (60/40 of the atomic density) on Si2 with
+    Gaussian smearing: the device SCF against the oracle's (free energy, both channels),
+    the moment relaxing to zero and the energy of the spin-unpolarised run."""
+    from amaterasu_amd.scf import SCFSettings, run_scf
+    from conftest import si_primitive_bohr
+    from oracle.scf import SCFSettings as OS
+    from oracle.scf import run_scf as orun
+    from oracle.system import build_system as obuild
+    from oracle.xc import XCFunctional
+
+    cell, pos = si_primitive_bohr()
+    s2 = obuild(cell, pos, ["Si", "Si"], {"Si": si_pseudo}, 8.0, 32.0, [[0.25, 0.25, 0.25], [-0.25, 0.25, 0.25]],
+                [0.25, 0.75], nspin=2)
+    s2.xc = XCFunctional("PBE")
+    rho = si2_system.initial_density()[0]
+    start = np.stack([0.6 * rho, 0.4 * rho])
+    kw = dict(tol_energy=1e-9, tol_density=1e-8, max_steps=60, scheme="gaussian", smearing=0.01)
+    got = run_scf(s2, SCFSettings(**kw), density=start)
+    ref = orun(s2, OS(**kw), density=start)
+    assert got.converged and ref.converged
+    assert abs(got.energies.free_energy - ref.energies.free_energy) < 1e-8
+    assert np.abs(got.density - ref.density).max() / ref.density.max() < 1e-6
+    assert np.abs(got.density[0] - got.density[1]).max() < 1e-6 * got.density.max()
+    s1 = _si2(si2_system, False)
+    one = run_scf(s1, SCFSettings(**kw))
+    assert abs(one.energies.free_energy - got.energies.free_energy) < 1e-8
