@@ -9,7 +9,7 @@ where ``rho_atomic`` is ``PlaneWaveSystem.initial_density`` (``hamiltonian/syste
 ``local.py:77-106``), f(r) are 16 Gaussian radial features of the atoms within 4 A of the
 grid point and y is a 16 -> 64 -> 64 -> 64 -> 1 SiLU network. The atomic superposition
 stays the fallback (no model, or a model whose start has a larger density residual) and
-the judge: the SCF converges to the same answer from either start; only the iteration
+the reference point: the SCF converges to the same answer from either start; only the iteration
 count and the wall time change.
 
 Features, inference and training (loss, gradient, Adam) are HIP kernels of

@@ -338,8 +338,8 @@ def main() -> int:
     ap.add_argument("--si214", action="store_true", help="add the Si214 k-grid SCF comparison (about ten minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the k-point/band sharding across "
-              "GPUs is described in DESIGN.md and not run here.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); sharding k-points and bands "
+              "across GPUs is not part of this measurement.", file=sys.stderr)
         return 2
 
     import numpy as np

@@ -13,5 +13,5 @@ dependencies ase/spglib/h5py are absent), so the oracle is pinned by the
 reference's own closed-form tests re-expressed in ``tests/test_oracle_*.py``
 (empty lattice, FFT normalisation, Parseval, Y_lm addition theorem, projector
 completeness, Davidson vs dense ``eigh``, ...). Functions that no such check
-pins are listed as "parity unpinned" in their docstrings and in DESIGN.md.
+pins are listed as "parity unpinned" in their docstrings.
 """
