@@ -11,6 +11,9 @@ output so the caller's data is untouched), records the fastest with ``amt_zgemm_
 and remembers the bucket for the rest of the process. The result of a product does not
 depend on the split beyond rounding.
 
+Products on the library's large-tile path (FP32 overlaps with both output sides >= 256)
+have their split fixed by accuracy; the hook says so and they are left alone.
+
 It is off by default: on the Si64 and Si8 shapes the heuristic was within a few per cent of
 the best candidate, and tuning the many buckets of a short run (band counts change as bands
 converge) costs more than it saves unless the process runs long enough to reuse them.
@@ -71,7 +74,11 @@ def tune_zgemm(run, prec: int, op: int, m: int, n: int, k: int, batch: int) -> i
     stream = torch.cuda.current_stream()
     candidates = {min(kmax, 256, max(1, int(round(h * f)))) for f in NEIGHBOURS} - {h}
     for split in [0] + sorted(candidates):
-        check(lib().amt_zgemm_set_split(prec, op, m, n, k, batch, split), "zgemm_set_split")
+        rc = lib().amt_zgemm_set_split(prec, op, m, n, k, batch, split)
+        if rc == 2:  # a large-tile product: its split is fixed, nothing to tune
+            _TUNED[key] = 0
+            return 0
+        check(rc, "zgemm_set_split")
         run()  # warm-up (and workspace growth)
         start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         start.record(stream)

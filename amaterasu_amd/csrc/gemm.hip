@@ -278,11 +278,14 @@ static int choose_split(int prec, int op, int m, int n, int k, int batch) {
   return (int)want;
 }
 
+static bool takes_large_tiles(int prec, int op, int m, int n, int k, int batch);
+
 int set_zgemm_split(int prec, int op, int m, int n, int k, int batch, int split) {
   if (split < 0 || split > 256) {
     set_error("zgemm split must be 0 (heuristic) .. 256");
     return 1;
   }
+  if (takes_large_tiles(prec, op, m, n, k, batch)) return 2;  // K chunks fixed by accuracy
   std::lock_guard<std::mutex> lock(g_split_mutex);
   const auto key = split_key(prec, op, m, n, k, batch);
   if (split == 0)
@@ -322,6 +325,10 @@ static GemmPlan plan_gemm(int prec, int op, int m, int n, int k, int batch) {
     if (batch * ns <= 65535 && bytes <= kBigWorkspace) return {2, (int)ns};
   }
   return {1, choose_split(prec, op, m, n, k, batch)};
+}
+
+static bool takes_large_tiles(int prec, int op, int m, int n, int k, int batch) {
+  return plan_gemm(prec, op, m, n, k, batch).tm == 2;
 }
 
 size_t zgemm_workspace_bytes(int prec, int op, int m, int n, int k, int batch) {

@@ -111,6 +111,8 @@ size_t amt_zgemm_workspace(int precision, int op, int m, int n, int k, int batch
  * (and amt_zgemm_workspace) use for every product in the shape's bucket (precision, op,
  * ceil(m/64), ceil(n/64), ceil(k/256), batch); split 0 restores the built-in heuristic.
  * For callers that time the candidates themselves (amaterasu_amd.autotune does).
+ * Returns 2 and records nothing for a product on the 128 x 128-tile path (FP32, op = 1,
+ * m and n >= 256), whose split is fixed by its K-chunk bound.
  */
 int amt_zgemm_set_split(int precision, int op, int m, int n, int k, int batch, int split);
 int amt_zgemm(int precision, int op, int m, int n, int k, double alpha, const void* a, long long lda, long long sa,

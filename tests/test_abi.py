@@ -45,3 +45,16 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "libamaterasu_amd.so")
     with pytest.raises(_lib.AmtError, match="not built"):
         _lib.lib()
+
+
+def test_split_hook_reports_large_tile_products():
+    """amt_zgemm_set_split (a host-side table, no GPU needed) records a split for 64 x 64-tile
+    products and answers 2 for the large-tile path, whose K chunks are fixed."""
+    from amaterasu_amd._lib import FP32, FP64, lib
+
+    assert lib().amt_zgemm_set_split(FP32, 1, 512, 512, 100000, 1, 4) == 2
+    assert lib().amt_zgemm_set_split(FP32, 0, 512, 512, 100000, 1, 4) == 0
+    assert lib().amt_zgemm_set_split(FP64, 1, 512, 512, 100000, 1, 4) == 0
+    assert lib().amt_zgemm_set_split(FP32, 0, 512, 512, 100000, 1, 0) == 0
+    assert lib().amt_zgemm_set_split(FP64, 1, 512, 512, 100000, 1, 0) == 0
+    assert lib().amt_zgemm_set_split(FP32, 1, 100, 512, 100000, 1, 300) == 1
