@@ -111,3 +111,17 @@ def test_occupation_errors_follow_the_reference():
         occupations(eig[0], w, 4.0, "gaussian", 0.01, 1)
     with pytest.raises(NotImplementedError):
         occupations(eig, w, 4.0, "tetrahedron", 0.01, 1)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says so, before touching
+    a device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
