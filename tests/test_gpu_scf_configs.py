@@ -1,0 +1,320 @@
+"""The workloads' own sizes on the GPU path, against the oracle, plus the reference's
+equation-of-state gate.
+
+* Config 1 at its stated size: diamond Si2, ecut 15 Ha, diamond symmetry (grid 28^3), the
+  two irreducible k-points of the shifted 2x2x2 mesh, 8 bands, fixed occupations, SCF
+  against the oracle's restatement of ``scf/driver.py:267-464``.
+* Si64 (84^3, npw 37 073, 1152 projectors): H.psi in FP64 and FP32 against the oracle's
+  ``Hamiltonian.apply`` (``hamiltonian/operator.py:60-69``) and the density against its
+  ``compute_density`` (``scf/density.py:35-84``), on the compile-time 84 kernels.
+* Si214 at Gamma (108^3, npw ~81k): the local part (projectors empty) on the FP32 plane
+  kernels and on the FP64 full-grid route, and the projector product at its full
+  440 x 3852 x 80 989 shape, checked on a slice against numpy FP64 matmul.
+* Si PBE E(V) at the DojoReport volumes (``tests/physics/test_dojo_eos.py:64-170`` in the
+  reference): V0, B0, B1 of a third-order Birch-Murnaghan fit, the delta gauge against the
+  report's own curve and the spread of the energy offset.
+
+The helpers :func:`fit_eos` and :func:`delta_gauge` are also checked on the CPU against the
+report's published fit (``tests/test_host_eos_fixture.py``).
+"""
+
+from __future__ import annotations
+
+import json
+import types
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, si_primitive_bohr
+
+#: 1 eV/A^3 in GPa (CODATA 2014 elementary charge, as the reference's units derive it).
+EV_PER_A3_IN_GPA = 1.6021766208e-19 / 1e-30 / 1e9
+
+
+# --------------------------------------------------------------------------- EOS helpers
+def birch_murnaghan(volume, e0, v0, b0, b1):
+    """Third-order Birch-Murnaghan E(V); ``b0`` in energy per volume units."""
+    x = (v0 / np.asarray(volume, float)) ** (2.0 / 3.0)
+    return e0 + 9.0 * v0 * b0 / 16.0 * ((x - 1.0) ** 3 * b1 + (x - 1.0) ** 2 * (6.0 - 4.0 * x))
+
+
+def fit_eos(volumes, energies):
+    """Least-squares third-order Birch-Murnaghan fit: ([E0, V0, B0, B1], rms residual).
+
+    The BM3 family is exactly the cubics in t = V^(-2/3), so the fit is a linear least-squares
+    problem in t; the equilibrium is the root of dE/dt = 0 with d2E/dt2 > 0, and with
+    t' = -(2/3) t/V the bulk modulus and its pressure derivative follow in closed form:
+    B0 = (4/9) t0^2 E_tt / V0 and B1 = 4 + (2/3) t0 E_ttt / E_tt.
+    """
+    v = np.asarray(volumes, float)
+    e = np.asarray(energies, float)
+    t = v ** (-2.0 / 3.0)
+    scale = t.mean()
+    u = t / scale  # well-conditioned abscissa
+    coef, *_ = np.linalg.lstsq(np.vander(u, 4), e, rcond=None)  # e = c3 u^3 + c2 u^2 + c1 u + c0
+    c3, c2, c1, c0 = coef
+    roots = np.roots([3.0 * c3, 2.0 * c2, c1]) if abs(c3) > 0 else np.array([-c1 / (2.0 * c2)])
+    roots = roots[np.isreal(roots)].real
+    curv = 6.0 * c3 * roots + 2.0 * c2
+    roots = roots[curv > 0]
+    if len(roots) == 0:
+        raise ValueError("the energies have no minimum")
+    u0 = roots[np.argmin(np.abs(roots - 1.0))]
+    e_uu = 6.0 * c3 * u0 + 2.0 * c2
+    e_uuu = 6.0 * c3
+    v0 = (u0 * scale) ** (-1.5)
+    e0 = ((c3 * u0 + c2) * u0 + c1) * u0 + c0
+    b0 = 4.0 / 9.0 * u0**2 * e_uu / v0  # d/dt = (1/scale) d/du, t = u scale: scales cancel
+    b1 = 4.0 + 2.0 / 3.0 * u0 * e_uuu / e_uu
+    params = np.array([e0, v0, b0, b1])
+    residual = float(np.sqrt(np.mean((birch_murnaghan(v, *params) - e) ** 2)))
+    return params, residual
+
+
+def delta_gauge(a, b, npoints: int = 2001) -> float:
+    """Lejaeghere's delta (Science 351, aad3000) between two fitted curves, in meV/atom: the
+    rms difference of E(V) - E(V0), each curve relative to its own minimum, over +-6 % of the
+    mean equilibrium volume (a constant offset between the curves does not enter)."""
+    vm = 0.5 * (a[1] + b[1])
+    grid = np.linspace(0.94 * vm, 1.06 * vm, npoints)
+    da = birch_murnaghan(grid, 0.0, *a[1:])  # zero at V = a[1]
+    db = birch_murnaghan(grid, 0.0, *b[1:])
+    diff2 = (da - db) ** 2
+    mean = float(np.sum(0.5 * (diff2[1:] + diff2[:-1]) * np.diff(grid)) / (grid[-1] - grid[0]))
+    return 1000.0 * np.sqrt(mean)
+
+
+# --------------------------------------------------------------------------- systems
+def _si2_symmetric(si_pseudo, a_ang, ecut, mesh, shift):
+    from oracle.symmetry import find_symmetry, ibz_kpoints
+    from oracle.system import build_system
+    from oracle.xc import XCFunctional
+
+    cell, pos = si_primitive_bohr(a_ang)
+    sym = find_symmetry(cell, pos @ np.linalg.inv(cell), ["Si", "Si"])
+    k, w = ibz_kpoints(sym, mesh, shift)
+    s = build_system(cell, pos, ["Si", "Si"], {"Si": si_pseudo}, ecut, 4.0 * ecut, k, w, symmetry=sym,
+                     xc=XCFunctional("PBE"))
+    return s
+
+
+def _rel(a, b) -> float:
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+
+# --------------------------------------------------------------------------- config 1
+@pytest.mark.gpu
+def test_config1_scf_at_its_size_matches_oracle(si_pseudo):
+    """Si2 at 15 Ha with diamond symmetry: grid 28^3, 2 irreducible k-points of weights 1/4
+    and 3/4, 8 bands; SCF to 1e-8 Ha against the oracle's."""
+    from amaterasu_amd.scf import HARTREE_IN_EV, SCFSettings, run_scf
+    from oracle.scf import SCFSettings as OS
+    from oracle.scf import run_scf as orun
+
+    s = _si2_symmetric(si_pseudo, 5.431, 15.0, [2, 2, 2], [0.5, 0.5, 0.5])
+    assert s.grid.shape == (28, 28, 28)
+    assert len(s.kpoints) == 2 and sorted(np.round(s.kpoints.weights, 12)) == [0.25, 0.75]
+    assert s.nbands == 8 and [len(sp) for sp in s.wave_spheres] in ([754, 748], [748, 754])
+    tol = 1e-8 * HARTREE_IN_EV / 2  # 1e-8 Ha per cell, in eV/atom
+    got = run_scf(s, SCFSettings(tol_energy=tol, tol_density=1e-9, max_steps=60))
+    ref = orun(s, OS(tol_energy=tol, tol_density=1e-9, max_steps=60))
+    assert got.converged and ref.converged
+    print(f"config 1: E {got.energies.total:.12f} vs oracle {ref.energies.total:.12f} Ha, "
+          f"{got.iterations}/{ref.iterations} iterations", flush=True)
+    assert abs(got.energies.total - ref.energies.total) < 1e-8
+    assert np.abs(got.eigenvalues - ref.eigenvalues).max() < 1e-7
+    assert got.electron_count == pytest.approx(8.0, abs=1e-9)
+    assert _rel(got.density, ref.density) < 1e-6
+
+
+# --------------------------------------------------------------------------- Si64 84^3
+@pytest.fixture(scope="module")
+def si64_oracle_system(si_pseudo):
+    from amaterasu_amd.workloads import si64_gamma
+    from oracle.system import build_system
+
+    spec = si64_gamma()
+    return build_system(spec.cell, spec.positions, spec.symbols, {"Si": si_pseudo}, float(spec.ecuts[0]),
+                        spec.ecut_rho, spec.kpoints, with_ewald=False)
+
+
+@pytest.mark.gpu
+def test_si64_hpsi_and_density_match_oracle(si64_oracle_system):
+    """The Si64 Gamma workload (84^3, npw 37 073, 1152 projectors): H.psi of 6 seeded bands on
+    the compile-time 84 stick/plane kernels and the MFMA projector GEMMs, FP64 to 1e-12 and
+    FP32 to 5e-6 of the oracle; the density of the same bands to 1e-12 (FP64)."""
+    import torch
+
+    from amaterasu_amd.density import compute_density
+    from amaterasu_amd.hamiltonian import Hamiltonian
+    from amaterasu_amd.workloads import seeded_coefficients, smooth_potential
+    from oracle.density import compute_density as ocd
+    from oracle.hamiltonian import Hamiltonian as OH
+
+    s = si64_oracle_system
+    assert s.grid.shape == (84, 84, 84) and len(s.wave_spheres[0]) == 37073
+    assert s.projectors[0].matrix.shape == (1152, 37073)
+    v = smooth_potential(s.grid.shape, seed=64)[None]
+    c = seeded_coefficients(s.wave_spheres[0].g2, 6, seed=640)
+    ref = OH(s, v).apply(c, 0)
+    h64 = Hamiltonian(s, v)
+    assert h64.plan.plane_ok[1]  # FP64 plane of 84^2 fits LDS: the pruned kernels run
+    e64 = _rel(h64.apply(c, 0), ref)
+    e32 = _rel(Hamiltonian(s, v, precision="fp32").apply(c, 0), ref)
+    print(f"Si64 H.psi rel err fp64 {e64:.2e} fp32 {e32:.2e}", flush=True)
+    assert e64 < 1e-12 and e32 < 5e-6
+    occ = np.array([[[2.0, 2.0, 1.5, 1.0, 0.5, 1e-13]]])
+    cn = c / np.linalg.norm(c, axis=1, keepdims=True)
+    rho, rho_ref = compute_density(s, [[cn]], occ), ocd(s, [[cn]], occ)
+    ed = _rel(rho, rho_ref)
+    # FP32 bands through the plane density kernel (FP64 sums) against the same oracle density
+    plan = h64.plan
+    psi = torch.zeros((1, 6, plan.npw_pad), dtype=torch.complex64, device="cuda")
+    psi[0, :, : plan.npw[0]] = torch.from_numpy(cn).to("cuda", torch.complex64)
+    acc = torch.zeros(s.grid.shape, dtype=torch.float64, device="cuda")
+    w = occ[0, 0, :5] * 1.0  # the oracle drops the 1e-13 occupation (scf/density.py:64-67)
+    plan.accumulate_density(psi, np.arange(5), w, acc, scale=1.0 / s.grid.volume)
+    ed32 = _rel(acc.cpu().numpy(), rho_ref[0])
+    print(f"Si64 density rel err fp64 {ed:.2e} fp32 bands {ed32:.2e}", flush=True)
+    assert ed < 1e-12 and ed32 < 5e-6
+
+
+# --------------------------------------------------------------------------- Si214 108^3
+@pytest.fixture(scope="module")
+def si214_local_system():
+    """Si214 at Gamma with the projectors left out: only the grid and the sphere matter to the
+    local part of H.psi."""
+    from amaterasu_amd.workloads import si214_cell
+    from oracle.basis import FFTGrid, GSphere
+    from oracle.pseudo import KBProjectors
+
+    cell, _ = si214_cell()
+    grid = FFTGrid.for_cutoff(cell, 60.0)
+    sphere = GSphere.build(cell, 15.0)
+    empty = KBProjectors(np.zeros((0, len(sphere)), dtype=complex), np.zeros((0, 0)), [])
+    return types.SimpleNamespace(grid=grid, wave_spheres=[sphere], projectors=[empty], nspin=1, paw=None)
+
+
+@pytest.mark.gpu
+def test_si214_local_hpsi_and_density_match_oracle(si214_local_system):
+    """108^3: the FP64 plane does not fit LDS, so FP64 H.psi and density take the full-grid
+    route; FP32 runs the compile-time 108 plane kernels. Both against the oracle."""
+    import torch
+
+    from amaterasu_amd.hamiltonian import Hamiltonian
+    from amaterasu_amd.workloads import seeded_coefficients, smooth_potential
+    from oracle.hamiltonian import Hamiltonian as OH
+
+    s = si214_local_system
+    assert s.grid.shape == (108, 108, 108) and 80000 < len(s.wave_spheres[0]) < 82000
+    v = smooth_potential(s.grid.shape, seed=214)[None]
+    c = seeded_coefficients(s.wave_spheres[0].g2, 3, seed=2140)
+    ref = OH(s, v).apply(c, 0)
+    h64 = Hamiltonian(s, v)
+    h32 = Hamiltonian(s, v, precision="fp32")
+    e64, e32 = _rel(h64.apply(c, 0), ref), _rel(h32.apply(c, 0), ref)
+    print(f"Si214 local H.psi rel err fp64 {e64:.2e} (plane kernels: {h64.plan.plane_ok[1]}) "
+          f"fp32 {e32:.2e} (plane kernels: {h32.plan.plane_ok[0]})", flush=True)
+    assert e64 < 1e-12 and e32 < 5e-6
+    # density of the same three bands, weights 2, 1, 0.5
+    grid = s.grid
+    cn = c / np.linalg.norm(c, axis=1, keepdims=True)
+    w = np.array([2.0, 1.0, 0.5])
+    u = grid.wave_to_real(cn, s.wave_spheres[0])
+    want = np.einsum("b,bxyz->xyz", w, np.abs(u) ** 2)
+    plan = h64.plan
+    for dt in (torch.complex128, torch.complex64):
+        psi = torch.zeros((1, 3, plan.npw_pad), dtype=dt, device="cuda")
+        psi[0, :, : plan.npw[0]] = torch.from_numpy(cn).to("cuda", dt)
+        acc = torch.zeros(grid.shape, dtype=torch.float64, device="cuda")
+        plan.accumulate_density(psi, np.arange(3), w, acc)
+        err = _rel(acc.cpu().numpy(), want)
+        print(f"Si214 density {dt} rel err {err:.2e}", flush=True)
+        assert err < (1e-12 if dt == torch.complex128 else 5e-6)
+
+
+@pytest.mark.gpu
+def test_si214_projector_gemm_shape_matches_numpy():
+    """P = psi beta^H at the Si214 shape (440 bands, 3852 projectors, K = 80 989 plane waves):
+    the whole product on the device (FP64 and FP32 MFMA kernels, with their split-K and
+    128x128-tile choices for this shape), checked on a 16 x 96 slice against numpy FP64."""
+    import torch
+
+    from amaterasu_amd import ops
+
+    nb, nproj, npw = 440, 3852, 80989
+    gen = torch.Generator(device="cuda").manual_seed(2140)
+    psi = torch.randn((1, nb, npw), dtype=torch.complex128, device="cuda", generator=gen)
+    beta = torch.randn((1, nproj, npw), dtype=torch.complex128, device="cuda", generator=gen)
+    rows = np.r_[0:8, nb - 8 : nb]
+    cols = np.r_[0:48, nproj - 48 : nproj]
+    a = psi[0, rows].cpu().numpy()
+    b = beta[0, cols].cpu().numpy()
+    want = a @ b.conj().T
+    p64 = ops.zgemm(psi, beta, 1)[0][rows][:, cols].cpu().numpy()
+    e64 = _rel(p64, want)
+    del p64
+    psi32, beta32 = psi.to(torch.complex64), beta.to(torch.complex64)
+    del psi, beta
+    p32 = ops.zgemm(psi32, beta32, 1)[0][rows][:, cols].cpu().numpy()
+    e32 = _rel(p32, want)
+    print(f"Si214 projector GEMM slice rel err fp64 {e64:.2e} fp32 {e32:.2e}", flush=True)
+    assert e64 < 1e-12
+    # FP32 inputs carry a 6e-8 relative rounding each; over K = 80 989 random terms the sum's
+    # relative error is that over sqrt(K)-scaled magnitudes: bound 1e-5 against the largest
+    assert e32 < 1e-5
+
+
+# --------------------------------------------------------------------------- E(V)
+@pytest.fixture(scope="module")
+def silicon_eos(si_pseudo):
+    """Device SCF total energies (eV/atom) at the DojoReport's seven volumes: PBE, ecut 18 Ha,
+    unshifted 8x8x8 mesh reduced by the diamond group, fixed occupations, tolerances 1e-9."""
+    from amaterasu_amd.scf import HARTREE_IN_EV, SCFSettings, run_scf
+
+    ref = json.loads((GOLDEN / "deltafactor_si.json").read_text())["Si"]
+    volumes = np.asarray(ref["volumes_A3"], float) / ref["num_sites"]
+    energies = []
+    for vol in volumes:
+        a = (8.0 * vol) ** (1.0 / 3.0)  # the primitive cell holds a^3/4 = 2 V
+        s = _si2_symmetric(si_pseudo, a, ref["ecut_Ha"], [8, 8, 8], [0.0, 0.0, 0.0])
+        r = run_scf(s, SCFSettings(tol_energy=1e-9, tol_density=1e-9, max_steps=60))
+        assert r.converged
+        energies.append(r.energies.extrapolated * HARTREE_IN_EV / 2)
+        print(f"E(V = {vol:.4f} A^3/atom) = {energies[-1]:.8f} eV/atom, {len(s.kpoints)} k-points, "
+              f"grid {s.grid.shape}, {r.iterations} iterations", flush=True)
+    return ref, volumes, np.asarray(energies)
+
+
+@pytest.mark.gpu
+def test_equation_of_state_matches_the_dojo_report(silicon_eos):
+    """``tests/physics/test_dojo_eos.py:109-124``: V0, B0, B1 within 0.2 %, 2 %, 5 % of the
+    published fit and a PBE lattice constant of 5.46-5.48 A."""
+    ref, volumes, energies = silicon_eos
+    p, residual = fit_eos(volumes, energies)
+    print(f"V0 {p[1]:.5f} A^3 B0 {p[2] * EV_PER_A3_IN_GPA:.3f} GPa B1 {p[3]:.4f} (published {ref['v0_A3_per_atom']:.5f}, "
+          f"{ref['b0_GPa']:.3f}, {ref['b1']:.4f}); fit rms {residual * 1000:.4f} meV", flush=True)
+    assert residual * 1000 < 0.05
+    assert p[1] == pytest.approx(ref["v0_A3_per_atom"], rel=2e-3)
+    assert p[2] * EV_PER_A3_IN_GPA == pytest.approx(ref["b0_GPa"], rel=0.02)
+    assert p[3] == pytest.approx(ref["b1"], rel=0.05)
+    assert 5.46 < (8.0 * p[1]) ** (1.0 / 3.0) < 5.48
+
+
+@pytest.mark.gpu
+def test_delta_gauge_and_offset_against_the_dojo_report(silicon_eos):
+    """``tests/physics/test_dojo_eos.py:127-170``: the delta gauge against the report's curve
+    below half the dataset's published delta (or 0.1 meV), and the energy offset constant
+    across the volumes (mean < 5 meV, spread < 0.1 meV)."""
+    ref, volumes, energies = silicon_eos
+    mine, _ = fit_eos(volumes, energies)
+    per_atom = np.asarray(ref["etotals_eV"], float) / ref["num_sites"]
+    theirs, _ = fit_eos(volumes, per_atom)
+    delta = delta_gauge(mine, theirs)
+    diff = energies - per_atom
+    print(f"delta {delta:.4f} meV/atom; offset mean {np.mean(diff) * 1000:.3f} meV spread "
+          f"{np.std(diff) * 1000:.4f} meV", flush=True)
+    assert delta < 0.5 * ref["dfact_meV"] or delta < 0.1
+    assert abs(float(np.mean(diff))) * 1000 < 5.0
+    assert float(np.std(diff)) * 1000 < 0.1
