@@ -173,7 +173,8 @@ def test_si64_hpsi_and_density_match_oracle(si64_oracle_system):
     psi = torch.zeros((1, 6, plan.npw_pad), dtype=torch.complex64, device="cuda")
     psi[0, :, : plan.npw[0]] = torch.from_numpy(cn).to("cuda", torch.complex64)
     acc = torch.zeros(s.grid.shape, dtype=torch.float64, device="cuda")
-    w = occ[0, 0, :5] * 1.0  # the oracle drops the 1e-13 occupation (scf/density.py:64-67)
+    # factors (2/nspin) w_k f with w_k = 1; the oracle drops the 1e-13 occupation (scf/density.py:64-67)
+    w = 2.0 * occ[0, 0, :5]
     plan.accumulate_density(psi, np.arange(5), w, acc, scale=1.0 / s.grid.volume)
     ed32 = _rel(acc.cpu().numpy(), rho_ref[0])
     print(f"Si64 density rel err fp64 {ed:.2e} fp32 bands {ed32:.2e}", flush=True)
