@@ -17,7 +17,9 @@
 //    b. eigenvectors by inverse iteration with each eigenvalue as its own shift (tridiagonal
 //       LU with partial pivoting, pivots floored at eps ||T||, as LAPACK's dlagtf/dlagts),
 //       three rounds from pseudo-random starts. Eigenvalues closer than kClusterGap ||T||
-//       to a neighbour form a cluster, solved by one workgroup: after every round the
+//       to a neighbour form a cluster, solved by one workgroup (a chain longer than
+//       kClusterMax is cut at its largest gaps into pieces that the workgroup solves in
+//       order, each orthogonalised against the piece before): after every round the
 //       cluster's vectors are orthonormalised together (classical Gram-Schmidt, twice), and
 //       a Rayleigh-Ritz step inside the cluster (cyclic Jacobi on the c x c projection)
 //       makes them eigenvectors, so near-degenerate and degenerate eigenvalues come out
@@ -738,7 +740,7 @@ __global__ __launch_bounds__(64) void k_tri_inverse(const double* __restrict__ d
 struct TriShared {
   double K[eig::kClusterMax][eig::kClusterMax + 1];
   double Q[eig::kClusterMax][eig::kClusterMax + 1];
-  double dots[eig::kClusterMax];
+  double dots[2 * eig::kClusterMax];  // a piece's rows and the previous piece's
   double red[eig::NT / 64 + 1];
   int order[eig::kClusterMax];
 };
@@ -789,45 +791,34 @@ __device__ bool orthonormalize_row(double* X, int n, int t, TriShared& sh) {
   return true;
 }
 
-// After each inverse-iteration round: workgroup k leads the cluster starting at k (if one
-// does) and orthonormalises its rows; after the last round it also solves the Rayleigh-Ritz
-// problem inside the cluster and writes the eigenvalues w[k] for k < nev.
-__global__ __launch_bounds__(eig::NT) void k_tri_cluster(const double* __restrict__ d, const double* __restrict__ e,
-                                                         int n, int m, int nev, int round, int last,
-                                                         const double* __restrict__ wb,
-                                                         const double* __restrict__ scal, double* __restrict__ w,
-                                                         double* __restrict__ S, double* __restrict__ work) {
-  __shared__ TriShared sh;
-  const int k = blockIdx.x;
-  if (k >= nev) return;
-  const double tnorm = scal[0];
-  const double delta = eig::kClusterGap * tnorm;
-  int s0 = k;
-  while (s0 > 0 && wb[s0] - wb[s0 - 1] < delta) --s0;
-  if ((k - s0) % eig::kClusterMax != 0) return;  // not the first member of a cluster
-  int k1 = k + 1;
-  while (k1 < m && k1 - k < eig::kClusterMax && wb[k1] - wb[k1 - 1] < delta) ++k1;
-  const int c = k1 - k;
-  double* X = S + (long long)k * n;
+// One piece of a chain of close eigenvalues, rows [ps, ps + c) of S, in one workgroup. X
+// points at the row `off` before the piece: rows 0..off-1 of X are the previous piece of the
+// same chain (already orthonormal), so the Gram-Schmidt pass makes the piece orthogonal to
+// them as well as orthonormal in itself. After the last round a Rayleigh-Ritz step inside
+// the piece makes its rows eigenvectors and writes their eigenvalues w[ps + t] (< nev).
+__device__ void solve_piece(const double* __restrict__ d, const double* __restrict__ e, int n, int nev, int round,
+                            int last, const double* __restrict__ wb, double tnorm, double* __restrict__ w,
+                            double* X, int off, int ps, int c, double* __restrict__ tmp, TriShared& sh) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-
-  for (int t = 0; t < c; ++t) {
+  for (int t = off; t < off + c; ++t) {
     for (int attempt = 0; attempt < 4; ++attempt) {
       if (orthonormalize_row(X, n, t, sh)) break;
       // dependent on the earlier rows: restart this row from fresh pseudo-random values
       double* xt = X + (long long)t * n;
       for (int i = tid; i < n; i += eig::NT)
-        xt[i] = start_value(((unsigned long long)(k + t) << 32) ^ (unsigned long long)i ^
+        xt[i] = start_value(((unsigned long long)(ps + t - off) << 32) ^ (unsigned long long)i ^
                             ((unsigned long long)(attempt + 1 + 4 * round) << 56));
       __syncthreads();
     }
   }
   if (!last) return;
   if (c == 1) {
-    if (tid == 0) w[k] = wb[k];
+    if (tid == 0 && ps < nev) w[ps] = wb[ps];
+    __syncthreads();
     return;
   }
-  // Rayleigh-Ritz inside the cluster: K = X T X^T (c x c), one entry pair per wave step
+  double* P = X + (long long)off * n;  // the piece's own rows
+  // Rayleigh-Ritz inside the piece: K = P T P^T (c x c), one entry pair per wave step
   const int npair = c * (c + 1) / 2;
   for (int pidx = wave; pidx < npair; pidx += eig::NT / 64) {
     int a = 0, rem = pidx;
@@ -836,8 +827,8 @@ __global__ __launch_bounds__(eig::NT) void k_tri_cluster(const double* __restric
       ++a;
     }
     const int b = a + rem;
-    const double* xa = X + (long long)a * n;
-    const double* xb = X + (long long)b * n;
+    const double* xa = P + (long long)a * n;
+    const double* xb = P + (long long)b * n;
     double s = 0.0;
     for (int i = lane; i < n; i += 64) {
       double tb = d[i] * xb[i];
@@ -903,26 +894,66 @@ __global__ __launch_bounds__(eig::NT) void k_tri_cluster(const double* __restric
         int best = t;
         for (int u = t + 1; u < c; ++u)
           if (sh.K[sh.order[u]][sh.order[u]] < sh.K[sh.order[best]][sh.order[best]]) best = u;
-        const int tmp = sh.order[t];
+        const int tmpo = sh.order[t];
         sh.order[t] = sh.order[best];
-        sh.order[best] = tmp;
+        sh.order[best] = tmpo;
       }
     }
   }
   __syncthreads();
-  // rotated vectors into the (no longer needed) factor storage of the cluster's rows, then back
-  double* tmp = work + 5LL * n * k;
+  // rotated vectors into the (no longer needed) factor storage of the piece's rows, then back
   for (int idx = tid; idx < c * n; idx += eig::NT) {
     const int t = idx / n, i = idx - t * n;
     const int o = sh.order[t];
     double v = 0.0;
-    for (int a = 0; a < c; ++a) v += sh.Q[a][o] * X[(long long)a * n + i];
+    for (int a = 0; a < c; ++a) v += sh.Q[a][o] * P[(long long)a * n + i];
     tmp[idx] = v;
   }
   __syncthreads();
-  for (int idx = tid; idx < c * n; idx += eig::NT) X[idx] = tmp[idx];
+  for (int idx = tid; idx < c * n; idx += eig::NT) P[idx] = tmp[idx];
   for (int t = tid; t < c; t += eig::NT)
-    if (k + t < nev) w[k + t] = sh.K[sh.order[t]][sh.order[t]];
+    if (ps + t < nev) w[ps + t] = sh.K[sh.order[t]][sh.order[t]];
+  __syncthreads();
+}
+
+// After each inverse-iteration round: workgroup k leads the chain of close eigenvalues that
+// starts at k (if one does; an isolated eigenvalue is a chain of one). A chain of at most
+// kClusterMax members is one piece. A longer chain is cut into pieces of at most kClusterMax,
+// each cut placed at the largest gap inside the next kClusterMax members, and the pieces are
+// solved in order by this workgroup, each orthogonalised against the piece before it, so
+// (near-)degenerate eigenvalues never straddle a cut and neighbouring pieces stay orthogonal.
+__global__ __launch_bounds__(eig::NT) void k_tri_cluster(const double* __restrict__ d, const double* __restrict__ e,
+                                                         int n, int m, int nev, int round, int last,
+                                                         const double* __restrict__ wb,
+                                                         const double* __restrict__ scal, double* __restrict__ w,
+                                                         double* __restrict__ S, double* __restrict__ work) {
+  __shared__ TriShared sh;
+  const int k = blockIdx.x;
+  if (k >= nev) return;
+  const double tnorm = scal[0];
+  const double delta = eig::kClusterGap * tnorm;
+  if (k > 0 && wb[k] - wb[k - 1] < delta) return;  // not the first member of a chain
+  int ce = k + 1;
+  while (ce < m && wb[ce] - wb[ce - 1] < delta) ++ce;
+  int prev = k;  // first row of the previous piece (= ps for the first piece)
+  for (int ps = k; ps < ce;) {
+    int pe = ce;
+    if (ce - ps > eig::kClusterMax) {  // cut before the largest gap in (ps, ps + kClusterMax]
+      pe = ps + 1;
+      double gbest = -1.0;
+      for (int q = ps + 1; q <= ps + eig::kClusterMax; ++q) {
+        const double g = wb[q] - wb[q - 1];
+        if (g >= gbest) {  // ties go to the later gap: the largest piece
+          gbest = g;
+          pe = q;
+        }
+      }
+    }
+    solve_piece(d, e, n, nev, round, last, wb, tnorm, w, S + (long long)prev * n, ps - prev, ps, pe - ps,
+                work + 5LL * n * ps, sh);
+    prev = ps;
+    ps = pe;
+  }
 }
 
 // ------------------------------------------------------------------ host

@@ -80,3 +80,42 @@ def test_batched_frozen_kpoint_keeps_its_answer():
     assert res.iterations[0] == 1
     assert res.iterations[1:].min() > 1
     assert torch.abs(res.eigenvalues[0] - pre.eigenvalues).max().item() < 1e-12
+
+
+def test_library_eigensolver_inside_davidson(monkeypatch):
+    """The library's Hermitian eigensolver inside the Davidson loop (projected matrices of
+    16..64 rows sent to it by lowering the size threshold): the per-k Davidson and the
+    k-batched one, whose per-k solves then run concurrently on side streams with their own
+    workspaces, against the per-k Davidson on torch.linalg.eigh. Eigenvalues to 1e-10,
+    orthonormal coefficients, and residual norms below the tolerance."""
+    from amaterasu_amd import eigen
+    from amaterasu_amd.batched import solve_batched
+    from amaterasu_amd.eigen import davidson, random_bands
+
+    s, ham = _ragged_system()
+    plan = ham.plan
+    nb, tol = 16, 1e-6
+    guess = torch.zeros((plan.nk, nb, plan.npw_pad), dtype=torch.complex128, device="cuda")
+    for k, sph in enumerate(s.wave_spheres):
+        guess[k, :, : len(sph.g2)] = random_bands(len(sph.g2), nb, sph.g2 / 2, seed=21 + k, device=True)
+    kin = plan.kinetic[1]
+    ref = []
+    for k in range(plan.nk):
+        n = plan.npw[k]
+        r = davidson(lambda b, k=k: ham.apply(b, k), guess[k, :, :n].contiguous(), kin[k, :n], tolerance=tol,
+                     max_iterations=100)
+        assert r.converged
+        ref.append(r)
+    monkeypatch.setattr(eigen, "HEEV_MIN", 16)
+    res = solve_batched(lambda b: ham.apply_batched(b), guess, kin, plan.npw, tolerance=tol, max_iterations=100)
+    assert res.converged.all()
+    for k in range(plan.nk):
+        n = plan.npw[k]
+        one = davidson(lambda b, k=k: ham.apply(b, k), guess[k, :, :n].contiguous(), kin[k, :n], tolerance=tol,
+                       max_iterations=100)
+        assert one.converged and float(one.residual_norms.max()) <= tol
+        for got in (one.eigenvalues, res.eigenvalues[k]):
+            assert torch.abs(got - ref[k].eigenvalues).max().item() < 1e-10
+        for c in (one.coefficients, res.coefficients[k][:, :n]):
+            ov = c @ c.conj().T
+            assert torch.abs(ov - torch.eye(nb, dtype=ov.dtype, device=ov.device)).max().item() < 1e-11

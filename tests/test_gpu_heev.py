@@ -73,3 +73,41 @@ def test_heev_repeated_calls_replay_correctly():
             assert np.abs(w.cpu().numpy() - np.linalg.eigvalsh(a)[:30]).max() < 1e-12 * n
             z = zt.cpu().numpy().T
             assert np.abs(a @ z - z * w.cpu().numpy()).max() < 1e-11 * n
+
+
+def _chain_matrix(n, seed, chain, start, spacing, degenerate_at):
+    """Hermitian matrix with a chain of `chain` eigenvalues whose neighbours are closer than
+    the solver's cluster threshold (2e-4 ||T||), an exactly degenerate pair at chain
+    positions degenerate_at and degenerate_at + 1, and well-separated eigenvalues around it."""
+    rng = np.random.default_rng(seed)
+    below = -1.0 + 0.01 * np.arange(start)
+    steps = spacing(rng, chain - 1)
+    steps[degenerate_at] = 0.0
+    mid = below[-1] + 0.01 + np.concatenate([[0.0], np.cumsum(steps)])
+    above = mid[-1] + 0.01 * (1 + np.arange(n - start - chain))
+    lam = np.concatenate([below, mid, above])
+    q, _ = np.linalg.qr(rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    return (q * lam) @ q.conj().T
+
+
+@pytest.mark.parametrize("spacing", ["irregular", "uniform", "all-degenerate"])
+def test_heev_chain_longer_than_a_cluster(spacing):
+    """A chain of 100 close eigenvalues (more than the 64 one workgroup solves together) with
+    an exactly degenerate pair at chain positions 63 and 64: the eigenvectors stay orthonormal
+    and accurate across the cut the solver has to make. Also an evenly spaced chain and a
+    70-fold degenerate eigenvalue."""
+    from amaterasu_amd import ops
+
+    n, nev = 300, 200
+    if spacing == "irregular":
+        a = _chain_matrix(n, 5, 100, 50, lambda rng, k: rng.uniform(1e-5, 3e-5, size=k), 63)
+    elif spacing == "uniform":
+        a = _chain_matrix(n, 6, 100, 50, lambda rng, k: np.full(k, 2e-5), 63)
+    else:
+        a = _chain_matrix(n, 7, 70, 50, lambda rng, k: np.zeros(k), 0)
+    w, zt = ops.heev_lowest(torch.from_numpy(a).cuda(), nev)
+    w, z = w.cpu().numpy(), zt.cpu().numpy().T
+    ref = np.linalg.eigvalsh(a)[:nev]
+    assert np.abs(w - ref).max() < 1e-13 * 3
+    assert np.abs(a @ z - z * w).max() < 1e-12 * 3
+    assert np.abs(z.conj().T @ z - np.eye(nev)).max() < 1e-12 * 3
