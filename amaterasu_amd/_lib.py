@@ -29,6 +29,7 @@ SIGNATURES = {
     "amt_plan_nsticks": (_c_int, [_vp, _ip]),
     "amt_plan_stick_bytes": (_c_sz, [_vp, _c_int, _c_int]),
     "amt_hpsi_local": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
+    "amt_hpsi_local_noalloc": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp]),
     "amt_hpsi_stage": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "amt_density_part_bytes": (_c_sz, [_vp, _c_int, _c_int]),
     "amt_density": (_c_int, [_vp, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp,

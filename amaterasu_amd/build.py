@@ -18,7 +18,7 @@ OUT = HERE / "libamaterasu_amd.so"
 BUILD = HERE / "_build"
 ARCH = os.environ.get("AMT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["abi.cpp", "fft.hip", "fft_fast.hip", "gemm.hip", "dav.hip", "mlp.hip", "eig.hip"]
+SOURCES = ["abi.cpp", "fft.hip", "fft_fast.hip", "gemm.hip", "dav.hip", "mlp.hip", "eig.hip", "noalloc.hip"]
 LIBS: list = []
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed-FP32 instructions are disabled on purpose: plain float code only.

@@ -320,6 +320,21 @@ int amt_hpsi_local(const amt_plan_t* h, int precision, const void* psi, void* hp
   return launch_hpsi_local(p, precision, psi, hpsi, nb, k0, nkk, veff, sticks, (hipStream_t)stream);
 }
 
+int amt_hpsi_local_noalloc(const amt_plan_t* h, int precision, const void* psi, void* hpsi, int nb, int k0, int nkk,
+                           const void* veff, void* stream) {
+  if (!h || !psi || !hpsi || !veff) {
+    set_error("hpsi_local_noalloc: null argument");
+    return 1;
+  }
+  const Plan& p = h->p;
+  if (check_plane(p, precision)) return 1;
+  if (nb <= 0 || nkk <= 0 || k0 < 0 || k0 + nkk > p.nk) {
+    set_error("hpsi_local_noalloc: band/k range outside the plan");
+    return 1;
+  }
+  return hpsi_local_noalloc(p, precision, psi, hpsi, nb, k0, nkk, veff, (hipStream_t)stream);
+}
+
 int amt_hpsi_stage(const amt_plan_t* h, int precision, int stage, const void* psi, void* hpsi, int nb, int k0,
                    int nkk, const void* veff, void* sticks, void* stream) {
   if (stage < 1 || stage > 3) {

@@ -64,6 +64,10 @@ int launch_density_fast(const Plan& p, int prec, const void* psi, int nb, int k0
 // kb = k * nb + b for k in [k0, k0 + nkb / nb). `sticks` is nkb * n1 * smax complex.
 int launch_hpsi_local(const Plan& p, int prec, const void* psi, void* hpsi, int nb, int k0, int nkk,
                       const void* veff, void* sticks, hipStream_t s, int stage = 0);
+// launch_hpsi_local over the library's fixed scratch array (noalloc.hip).
+long long noalloc_rows_per_launch(const Plan& p, int prec);
+int hpsi_local_noalloc(const Plan& p, int prec, const void* psi, void* hpsi, int nb, int k0, int nkk,
+                       const void* veff, hipStream_t s);
 // sum over listed bands of w |u(r)|^2 into part[chunk][N] (double), then
 // rho = beta * rho + scale * sum_chunks part.
 int launch_density(const Plan& p, int prec, const void* psi, int nb, int k0, int nkk, const int* d_band_list,

@@ -66,6 +66,22 @@ size_t amt_plan_stick_bytes(const amt_plan_t* plan, int precision, int nbands);
 int amt_hpsi_local(const amt_plan_t* plan, int precision, const void* psi, void* hpsi, int nb, int k0, int nk,
                    const void* veff, void* sticks, void* stream);
 
+/*
+ * amt_hpsi_local without a workspace argument and without any device allocation. No
+ * counterpart in the reference: for callers that cannot allocate (code under stream or
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The stick
+ * workspace is a fixed per-device array of AMT_NOALLOC_SCRATCH_BYTES inside the library,
+ * held by each GPU that runs it until the process ends; rows go through it in launches of
+ * at most that many bytes of sticks (whole k-points, or band slices of one k-point), with
+ * results bit-identical to amt_hpsi_local. Fails with an error if a single band row needs
+ * more sticks than the array holds (n1 * smax * 8 bytes in fp32, 16 in fp64). All calls on
+ * one GPU share its array: calls that overlap on one device (other streams, other host
+ * threads) race; calls on one stream are ordered.
+ */
+#define AMT_NOALLOC_SCRATCH_BYTES (64u << 20)
+int amt_hpsi_local_noalloc(const amt_plan_t* plan, int precision, const void* psi, void* hpsi, int nb, int k0,
+                           int nk, const void* veff, void* stream);
+
 /* One kernel of amt_hpsi_local (stage 1: stick inverse, 2: plane V_eff, 3: stick forward),
  * for timing a single kernel; running 1, 2, 3 in order equals amt_hpsi_local. */
 int amt_hpsi_stage(const amt_plan_t* plan, int precision, int stage, const void* psi, void* hpsi, int nb, int k0,
