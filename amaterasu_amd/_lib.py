@@ -29,6 +29,9 @@ SIGNATURES = {
     "amt_plan_nsticks": (_c_int, [_vp, _ip]),
     "amt_plan_stick_bytes": (_c_sz, [_vp, _c_int, _c_int]),
     "amt_hpsi_local": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
+    "amt_plan_set_projectors": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp]),
+    "amt_hpsi_workspace": (_c_sz, [_vp, _c_int, _c_int, _c_int]),
+    "amt_hpsi": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_sz, _vp]),
     "amt_hpsi_local_noalloc": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp]),
     "amt_hpsi_stage": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "amt_density_part_bytes": (_c_sz, [_vp, _c_int, _c_int]),
@@ -36,7 +39,9 @@ SIGNATURES = {
                              _c_dbl, _c_dbl, _vp]),
     "amt_scatter": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp]),
     "amt_gather": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_dbl, _vp]),
-    "amt_fft3d": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _vp, _c_int, _c_dbl, _vp]),
+    "amt_fft_plan_create": (_c_int, [ctypes.POINTER(_vp), _c_int, _c_int, _c_int]),
+    "amt_fft_plan_destroy": (_c_int, [_vp]),
+    "amt_fft3d": (_c_int, [_vp, _c_int, _c_int, _vp, _c_int, _c_dbl, _vp]),
     "amt_zgemm_workspace": (_c_sz, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
     "amt_zgemm_set_split": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int]),
     "amt_zgemm": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_dbl, _vp, _c_ll, _c_ll, _vp, _c_ll, _c_ll,

@@ -41,11 +41,14 @@ static int upload(T** dst, const std::vector<T>& src, const char* what) {
 }
 
 static void free_plan_device(Plan& p) {
-  void* ptrs[] = {p.d_flatmap, p.d_stick_jk, p.d_stick_map, p.d_nsticks, p.d_rows,      p.d_nrows,  p.d_kin64,
-                  p.d_kin32,   p.d_tw64[0],  p.d_tw64[1],    p.d_tw64[2],  p.d_tw32[0], p.d_tw32[1], p.d_tw32[2]};
+  void* ptrs[] = {p.d_flatmap,  p.d_stick_jk, p.d_stick_map, p.d_nsticks, p.d_rows,       p.d_nrows,
+                  p.d_kin64,    p.d_kin32,    p.d_tw64[0],   p.d_tw64[1],  p.d_tw64[2],    p.d_tw32[0],
+                  p.d_tw32[1],  p.d_tw32[2],  p.d_rowmask,   p.d_rowstart, p.d_livemask};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   p.d_flatmap = p.d_stick_jk = p.d_stick_map = p.d_nsticks = p.d_rows = p.d_nrows = nullptr;
+  p.d_rowmask = p.d_livemask = nullptr;
+  p.d_rowstart = nullptr;
   p.d_kin64 = nullptr;
   p.d_kin32 = nullptr;
   for (int a = 0; a < 3; ++a) {
@@ -200,6 +203,28 @@ int amt_plan_create(amt_plan_t** out, int n1, int n2, int n3, int nk, const int*
       if (used[j]) rows[(size_t)k * n2 + nrows[k]++] = j;
   }
   p.fast = (n1 == n2 && n2 == n3 && fast_length_supported(n1));
+  // per-row stick masks and first-stick indices for the compile-time plane kernels
+  std::vector<unsigned long long> rowmask, livemask;
+  std::vector<int> rowstart;
+  if (n2 <= 128 && n3 <= 128) {
+    rowmask.assign((size_t)nk * n2 * 2, 0ull);
+    rowstart.assign((size_t)nk * n2, 0);
+    livemask.assign((size_t)nk * 2, 0ull);
+    for (int k = 0; k < nk; ++k) {
+      std::vector<int> count(n2, 0);
+      for (int v : sticks[k]) {
+        const int j = v / n3, kk = v % n3;
+        rowmask[((size_t)k * n2 + j) * 2 + (kk >> 6)] |= 1ull << (kk & 63);
+        livemask[(size_t)k * 2 + (j >> 6)] |= 1ull << (j & 63);
+        ++count[j];
+      }
+      int run = 0;
+      for (int j = 0; j < n2; ++j) {
+        rowstart[(size_t)k * n2 + j] = run;
+        run += count[j];
+      }
+    }
+  }
   std::vector<float> kin32(kin64.begin(), kin64.end());
   // launch geometry (must fit the LDS FFT's per-thread butterfly budget)
   const int lds_max = max_lds_bytes();
@@ -220,9 +245,9 @@ int amt_plan_create(amt_plan_t** out, int n1, int n2, int n3, int nk, const int*
     const long long plds = (long long)n2 * n3 * csz;
     // 0 marks a precision whose plane does not fit one workgroup on this device. The
     // compile-time-length kernels use a fixed 256-thread workgroup (512 for the density)
-    // and add the row list (n2 ints) to the plane.
+    // and add the row lists, first-stick indices and 128-bit stick masks (24 n2 bytes).
     if (p.fast)
-      p.plane_threads[prec] = (plds + 4LL * n2 <= lds_max) ? 256 : 0;
+      p.plane_threads[prec] = (plds + 24LL * n2 <= lds_max) ? 256 : 0;
     else
       p.plane_threads[prec] = (pt <= 1024 && plds <= lds_max) ? pt : 0;
   }
@@ -235,7 +260,10 @@ int amt_plan_create(amt_plan_t** out, int n1, int n2, int n3, int nk, const int*
   if (!upload(&p.d_flatmap, flat, "plan flatmap") || !upload(&p.d_stick_jk, jk, "plan sticks") ||
       !upload(&p.d_stick_map, smap, "plan stick map") || !upload(&p.d_nsticks, nst, "plan nsticks") ||
       !upload(&p.d_rows, rows, "plan rows") || !upload(&p.d_nrows, nrows, "plan nrows") ||
-      !upload(&p.d_kin64, kin64, "plan kinetic") || !upload(&p.d_kin32, kin32, "plan kinetic32")) {
+      !upload(&p.d_kin64, kin64, "plan kinetic") || !upload(&p.d_kin32, kin32, "plan kinetic32") ||
+      (!rowmask.empty() && (!upload(&p.d_rowmask, rowmask, "plan row masks") ||
+                            !upload(&p.d_rowstart, rowstart, "plan row starts") ||
+                            !upload(&p.d_livemask, livemask, "plan live rows")))) {
     free_plan_device(p);
     delete h;
     return 1;
@@ -318,6 +346,94 @@ int amt_hpsi_local(const amt_plan_t* h, int precision, const void* psi, void* hp
     return 1;
   }
   return launch_hpsi_local(p, precision, psi, hpsi, nb, k0, nkk, veff, sticks, (hipStream_t)stream);
+}
+
+int amt_plan_set_projectors(amt_plan_t* h, int precision, int nproj, const void* matrix, const void* dij,
+                            const int* blk_start, const int* blk_len) {
+  if (!h || (precision != kFP32 && precision != kFP64) || nproj < 0) {
+    set_error("plan_set_projectors: null plan, bad precision or nproj < 0");
+    return 1;
+  }
+  if (nproj > 0 && (!matrix || !dij || !blk_start || !blk_len)) {
+    set_error("plan_set_projectors: null table");
+    return 1;
+  }
+  Plan& p = h->p;
+  p.nproj[precision] = nproj;
+  p.kb_matrix[precision] = nproj > 0 ? matrix : nullptr;
+  p.kb_dij[precision] = nproj > 0 ? dij : nullptr;
+  p.kb_blk_start[precision] = nproj > 0 ? blk_start : nullptr;
+  p.kb_blk_len[precision] = nproj > 0 ? blk_len : nullptr;
+  return 0;
+}
+
+namespace {
+constexpr size_t kAlign = 256;
+size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
+struct HpsiLayout {
+  size_t sticks = 0, proj = 0, pd = 0, gemm = 0, total = 0;
+};
+HpsiLayout hpsi_layout(const Plan& p, int prec, int nb, int nkk) {
+  HpsiLayout l;
+  const size_t elem = prec == kFP64 ? 16 : 8;
+  l.sticks = align_up((size_t)nb * nkk * p.n1 * p.smax * elem);
+  const int np = p.nproj[prec];
+  if (np > 0) {
+    l.proj = align_up((size_t)nkk * nb * np * elem);
+    l.pd = l.proj;
+    const size_t g1 = zgemm_workspace_bytes(prec, 1, nb, np, p.npw_pad, nkk);
+    const size_t g2 = zgemm_workspace_bytes(prec, 0, nb, p.npw_pad, np, nkk);
+    l.gemm = align_up(g1 > g2 ? g1 : g2);
+  }
+  l.total = l.sticks + l.proj + l.pd + l.gemm;
+  return l;
+}
+}  // namespace
+
+size_t amt_hpsi_workspace(const amt_plan_t* h, int precision, int nb, int nkk) {
+  if (!h || (precision != kFP32 && precision != kFP64) || nb <= 0 || nkk <= 0) return 0;
+  return hpsi_layout(h->p, precision, nb, nkk).total;
+}
+
+int amt_hpsi(const amt_plan_t* h, int precision, const void* psi, void* hpsi, int nb, int k0, int nkk,
+             const void* veff, void* ws, size_t ws_bytes, void* stream) {
+  if (!h || !psi || !hpsi || !veff || !ws) {
+    set_error("hpsi: null argument");
+    return 1;
+  }
+  const Plan& p = h->p;
+  if (check_plane(p, precision)) return 1;
+  if (nb <= 0 || nkk <= 0 || k0 < 0 || k0 + nkk > p.nk || (long long)nb * nkk > 65535) {
+    set_error("hpsi: band/k range outside the plan");
+    return 1;
+  }
+  const HpsiLayout l = hpsi_layout(p, precision, nb, nkk);
+  if (ws_bytes < l.total) {
+    set_error("hpsi: workspace smaller than amt_hpsi_workspace()");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  unsigned char* base = static_cast<unsigned char*>(ws);
+  if (launch_hpsi_local(p, precision, psi, hpsi, nb, k0, nkk, veff, base, s)) return 1;
+  const int np = p.nproj[precision];
+  if (np == 0) return 0;
+  // V_NL psi = (P D) beta with P = psi beta^H (pseudo/kb.py:216-230)
+  const size_t elem = precision == kFP64 ? 16 : 8;
+  const unsigned char* mat = static_cast<const unsigned char*>(p.kb_matrix[precision]) + (size_t)k0 * np * p.npw_pad * elem;
+  const unsigned char* dij =
+      static_cast<const unsigned char*>(p.kb_dij[precision]) + (size_t)k0 * np * np * (precision == kFP64 ? 8 : 4);
+  void* proj = base + l.sticks;
+  void* pd = base + l.sticks + l.proj;
+  void* gws = base + l.sticks + l.proj + l.pd;
+  const long long spsi = (long long)nb * p.npw_pad, smat = (long long)np * p.npw_pad, sp = (long long)nb * np;
+  if (launch_zgemm(precision, 1, nb, np, p.npw_pad, 1.0, psi, p.npw_pad, spsi, mat, p.npw_pad, smat, 0.0, proj, np,
+                   sp, nkk, gws, l.gemm, s))
+    return 1;
+  if (launch_dij(precision, proj, dij, p.kb_blk_start[precision], p.kb_blk_len[precision], pd, nb, np, sp,
+                 (long long)np * np, sp, nkk, s))
+    return 1;
+  return launch_zgemm(precision, 0, nb, p.npw_pad, np, 1.0, pd, np, sp, mat, p.npw_pad, smat, 1.0, hpsi, p.npw_pad,
+                      spsi, nkk, gws, l.gemm, s);
 }
 
 int amt_hpsi_local_noalloc(const amt_plan_t* h, int precision, const void* psi, void* hpsi, int nb, int k0, int nkk,
@@ -406,12 +522,64 @@ int amt_gather(const amt_plan_t* h, int precision, const void* grid, void* out, 
   return launch_gather(h->p, precision, grid, out, nb, k, scale, (hipStream_t)stream);
 }
 
-int amt_fft3d(int precision, int n1, int n2, int n3, int batch, void* data, int sign, double scale, void* stream) {
-  if (!data || batch <= 0 || (sign != 1 && sign != -1)) {
+struct amt_fft_plan {
+  FftPlan f;
+};
+
+static void free_fft_plan(FftPlan& f) {
+  for (int a = 0; a < 3; ++a) {
+    if (f.tw64[a]) (void)hipFree(f.tw64[a]);
+    if (f.tw32[a]) (void)hipFree(f.tw32[a]);
+    f.tw64[a] = nullptr;
+    f.tw32[a] = nullptr;
+  }
+}
+
+int amt_fft_plan_create(amt_fft_plan_t** out, int n1, int n2, int n3) {
+  if (!out) {
+    set_error("fft_plan_create: null argument");
+    return 1;
+  }
+  *out = nullptr;
+  auto* h = new amt_fft_plan;
+  FftPlan& f = h->f;
+  const int dims[3] = {n1, n2, n3};
+  for (int a = 0; a < 3; ++a) {
+    f.n[a] = dims[a];
+    if (dims[a] <= 0 || !make_axis_plan(dims[a], &f.ap[a])) {
+      set_error("fft_plan_create: grid length " + std::to_string(dims[a]) + " is not 2,3,5,7,11-smooth");
+      delete h;
+      return 1;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    std::vector<double> tw;
+    twiddles(dims[a], tw);
+    std::vector<float> t32(tw.begin(), tw.end());
+    if (!upload(reinterpret_cast<double**>(&f.tw64[a]), tw, "fft plan twiddles") ||
+        !upload(reinterpret_cast<float**>(&f.tw32[a]), t32, "fft plan twiddles32")) {
+      free_fft_plan(f);
+      delete h;
+      return 1;
+    }
+  }
+  *out = h;
+  return 0;
+}
+
+int amt_fft_plan_destroy(amt_fft_plan_t* h) {
+  if (!h) return 0;
+  free_fft_plan(h->f);
+  delete h;
+  return 0;
+}
+
+int amt_fft3d(const amt_fft_plan_t* h, int precision, int batch, void* data, int sign, double scale, void* stream) {
+  if (!h || !data || batch <= 0 || (sign != 1 && sign != -1) || (precision != kFP32 && precision != kFP64)) {
     set_error("fft3d: bad argument");
     return 1;
   }
-  return launch_fft3d(precision, n1, n2, n3, batch, data, sign, scale, (hipStream_t)stream);
+  return launch_fft3d(h->f, precision, batch, data, sign, scale, (hipStream_t)stream);
 }
 
 size_t amt_zgemm_workspace(int precision, int op, int m, int n, int k, int batch) {

@@ -457,45 +457,15 @@ int launch_gather(const Plan& p, int prec, const void* grid, void* out, int nb, 
                        : gather_t<float>(p, grid, out, nb, k, scale, s);
 }
 
-// Twiddle tables for the standalone transform, cached per (precision, n) for the process.
-struct TwCache {
-  int n = 0;
-  void* d = nullptr;
-};
-static std::vector<TwCache> g_twcache[2];
-
-static const void* cached_twiddles(int prec, int n) {
-  for (auto& t : g_twcache[prec])
-    if (t.n == n) return t.d;
-  std::vector<double> tw;
-  twiddles(n, tw);
-  void* d = nullptr;
-  if (prec == kFP64) {
-    if (hipMalloc(&d, tw.size() * sizeof(double)) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-  } else {
-    std::vector<float> t32(tw.begin(), tw.end());
-    if (hipMalloc(&d, t32.size() * sizeof(float)) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-  }
-  g_twcache[prec].push_back({n, d});
-  return d;
-}
-
 template <typename R>
-static int fft3d_t(int n1, int n2, int n3, int batch, void* data_, int sign, double scale, hipStream_t s) {
+static int fft3d_t(const FftPlan& fp, int batch, void* data_, int sign, double scale, hipStream_t s) {
   using C = typename Cplx<R>::T;
   constexpr int prec = sizeof(R) == 8 ? kFP64 : kFP32;
   constexpr int TPT = sizeof(R) == 8 ? kTptF64 : kTptF32;
   C* data = static_cast<C*>(data_);
-  const int dims[3] = {n1, n2, n3};
-  AxisPlan ap[3];
-  for (int a = 0; a < 3; ++a) {
-    if (!make_axis_plan(dims[a], &ap[a])) {
-      set_error("fft3d: grid length " + std::to_string(dims[a]) + " is not 2,3,5,7,11-smooth");
-      return 1;
-    }
-  }
+  const int n1 = fp.n[0], n2 = fp.n[1], n3 = fp.n[2];
+  const int* dims = fp.n;
+  const AxisPlan* ap = fp.ap;
   // axis 2 (contiguous), then 1, then 0; scale folded into the last pass
   for (int step = 0; step < 3; ++step) {
     const int a = 2 - step;
@@ -508,11 +478,7 @@ static int fft3d_t(int n1, int n2, int n3, int batch, void* data_, int sign, dou
       set_error("fft3d: axis too long for one workgroup");
       return 1;
     }
-    const C* tw = static_cast<const C*>(cached_twiddles(prec, dims[a]));
-    if (!tw) {
-      set_error("fft3d: twiddle allocation failed");
-      return 1;
-    }
+    const C* tw = static_cast<const C*>(prec == kFP64 ? (const void*)fp.tw64[a] : (const void*)fp.tw32[a]);
     const long long nblk = inner == 1 ? (nouter + W - 1) / W : nouter * ((inner + W - 1) / W);
     const size_t lds = (size_t)dims[a] * W * sizeof(C);
     const R sc = (R)(step == 2 ? scale : 1.0);
@@ -527,9 +493,9 @@ static int fft3d_t(int n1, int n2, int n3, int batch, void* data_, int sign, dou
   return 0;
 }
 
-int launch_fft3d(int prec, int n1, int n2, int n3, int batch, void* data, int sign, double scale, hipStream_t s) {
-  return prec == kFP64 ? fft3d_t<double>(n1, n2, n3, batch, data, sign, scale, s)
-                       : fft3d_t<float>(n1, n2, n3, batch, data, sign, scale, s);
+int launch_fft3d(const FftPlan& fp, int prec, int batch, void* data, int sign, double scale, hipStream_t s) {
+  return prec == kFP64 ? fft3d_t<double>(fp, batch, data, sign, scale, s)
+                       : fft3d_t<float>(fp, batch, data, sign, scale, s);
 }
 
 }  // namespace amt

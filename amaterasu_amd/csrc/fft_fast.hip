@@ -33,17 +33,31 @@ constexpr int stick_width() {
 }
 
 // ----------------------------------------------------------------------- kernels
+// Stick kernels run on a 1-D grid of 8 * nch * ceil(nkb / 8) workgroups. Workgroup w goes to
+// XCD w % 8, and all nch stick chunks of band row kb get w % 8 = kb % 8, so every chunk of a
+// band row runs on one XCD and its scattered sphere-order reads and writes of that row meet in
+// one L2 instead of being fetched once per XCD.
+__device__ __forceinline__ bool stick_block(int nch, int nkb, int& chunk, int& kb) {
+  const int w = blockIdx.x, slot = w >> 3;
+  chunk = slot % nch;
+  kb = (slot / nch) * 8 + (w & 7);
+  return kb < nkb;
+}
+inline unsigned stick_grid(int nch, int nkb) { return 8u * (unsigned)nch * (unsigned)((nkb + 7) / 8); }
+
 template <int N, typename C>
 __global__ __launch_bounds__(256) void k_stick_inverse_n(const C* __restrict__ psi, C* __restrict__ sticks,
                                                          const int* __restrict__ stick_map, int smax, int npw_pad,
-                                                         int nb, int k0, const C* __restrict__ tw) {
+                                                         int nb, int k0, const C* __restrict__ tw, int nch,
+                                                         int nkb) {
   constexpr int W = stick_width<C>();
   constexpr int T2 = ceil_div(W * Split<N>::N1, 256);
   __shared__ C tile[N * W];
   const int tid = threadIdx.x;
-  const int kb = blockIdx.y;
+  int chunk, kb;
+  if (!stick_block(nch, nkb, chunk, kb)) return;
   const int k = k0 + kb / nb;
-  const int s0 = blockIdx.x * W;
+  const int s0 = chunk * W;
   const C* c = psi + (long long)kb * npw_pad;
   const int* mp = stick_map + (long long)k * N * smax;
   for (int e = tid; e < N * W; e += 256) {
@@ -68,14 +82,15 @@ template <int N, typename C, typename R>
 __global__ __launch_bounds__(256) void k_stick_forward_n(const C* __restrict__ sticks, const C* __restrict__ psi,
                                                          C* __restrict__ out, const int* __restrict__ stick_map,
                                                          const R* __restrict__ kin, int smax, int npw_pad, int nb,
-                                                         int k0, const C* __restrict__ tw, R scale) {
+                                                         int k0, const C* __restrict__ tw, R scale, int nch, int nkb) {
   constexpr int W = stick_width<C>();
   constexpr int T2 = ceil_div(W * Split<N>::N1, 256);
   __shared__ C tile[N * W];
   const int tid = threadIdx.x;
-  const int kb = blockIdx.y;
+  int chunk, kb;
+  if (!stick_block(nch, nkb, chunk, kb)) return;
   const int k = k0 + kb / nb;
-  const int s0 = blockIdx.x * W;
+  const int s0 = chunk * W;
   const C* in = sticks + (long long)kb * N * smax;
   for (int e = tid; e < N * W; e += 256) {
     const int i = e / W, w = e % W, s = s0 + w;
@@ -100,92 +115,353 @@ __global__ __launch_bounds__(256) void k_stick_forward_n(const C* __restrict__ s
   }
 }
 
+// ----------------------------------------------------------------------- plane pieces
+// Bit kk of a row's 128-bit stick mask, and the number of set bits below it.
+__device__ __forceinline__ bool mask_bit(unsigned long long lo, unsigned long long hi, int kk) {
+  return kk < 64 ? ((lo >> kk) & 1ull) : ((hi >> (kk - 64)) & 1ull);
+}
+__device__ __forceinline__ int mask_below(unsigned long long lo, unsigned long long hi, int kk) {
+  return kk < 64 ? __popcll(lo & ((1ull << kk) - 1ull)) : __popcll(lo) + __popcll(hi & ((1ull << (kk - 64)) - 1ull));
+}
+
+// u[k1] *= w_N^{SIGN j2 k1} (tw = exp(-2 pi i m / N); j2 k1 < N for the four-step indices).
+template <int N, int NA, int SIGN, typename C>
+__device__ __forceinline__ void twiddle_row(C (&u)[NA], int j2, const C* __restrict__ tw) {
+#pragma unroll
+  for (int k1 = 1; k1 < NA; ++k1) {
+    C w = tw[j2 * k1];
+    if (SIGN > 0) w.y = -w.y;
+    u[k1] = cmul(u[k1], w);
+  }
+}
+
+// Local part of H.psi for one i-plane of one band row: the inverse 2D transform of the
+// plane's sticks, times V_eff, and the forward 2D transform back onto the sticks. The length
+// N = N1 N2 transforms are four-step (fft_reg.hpp) with these savings over a plain pair of
+// 2D transforms:
+//  * the sticks are read straight from the stick workspace by the first phase along k (the
+//    row masks give each (j, k) its stick index), so the plane is never zeroed or scattered;
+//  * along k only the rows j that hold sticks are transformed, and the first phase along j
+//    reads the other rows as zeros;
+//  * the second inverse phase along j, the V_eff multiply and the first forward phase along j
+//    are one register pass: the forward transform uses the transposed split (N2 x N1), whose
+//    first phase consumes exactly the N2 outputs one inverse task holds;
+//  * the forward phase along j writes only rows that hold sticks, and the last phase along k
+//    writes its outputs straight back to the sticks.
 template <int N, typename C, typename R>
-__global__ __launch_bounds__(256) void k_plane_vloc_n(C* __restrict__ sticks, const int* __restrict__ stick_jk,
-                                                      const int* __restrict__ nsticks, const int* __restrict__ rows,
-                                                      const int* __restrict__ nrows, const R* __restrict__ veff,
-                                                      int smax, int nb, int k0, const C* __restrict__ tw) {
-  constexpr int NP = N * N;
-  constexpr int T2 = ceil_div(N * Split<N>::N1, 256);
+__global__ __launch_bounds__(256) void k_plane_vloc_n(C* __restrict__ sticks, const int* __restrict__ rows,
+                                                      const int* __restrict__ nrows,
+                                                      const unsigned long long* __restrict__ rowmask,
+                                                      const int* __restrict__ rowstart,
+                                                      const unsigned long long* __restrict__ livemask,
+                                                      const R* __restrict__ veff, int smax, int nb, int k0,
+                                                      const C* __restrict__ tw) {
+  constexpr int NTH = 256;
+  constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  constexpr int TR = ceil_div(N * N1, NTH);  // rounds of the (row or column, k1) tasks
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   C* plane = reinterpret_cast<C*>(smem_raw);
-  __shared__ int rl[(N + 3) / 4 * 4];  // a multiple of 16 bytes: the dynamic plane after it stays 16-byte aligned
+  __shared__ int rl[(N + 3) / 4 * 4];  // sizes in multiples of 16 bytes keep the dynamic plane 16-byte aligned
+  __shared__ int rs[(N + 3) / 4 * 4];
+  __shared__ unsigned long long rm[2 * N];
   const int tid = threadIdx.x;
   const int i = blockIdx.x;
   const int kb = blockIdx.y;
   const int k = k0 + kb / nb;
-  const int S = nsticks[k];
   const int nr = nrows[k];
-  const int* jk = stick_jk + (long long)k * smax;
+  const unsigned long long live_lo = livemask[2 * k], live_hi = livemask[2 * k + 1];
   C* st = sticks + ((long long)kb * N + i) * smax;
-  for (int e = tid; e < NP; e += 256) plane[e] = cmake<C>(0, 0);
-  for (int r = tid; r < nr; r += 256) rl[r] = rows[(long long)k * N + r];
+  for (int r = tid; r < nr; r += NTH) {
+    const int j = rows[(long long)k * N + r];
+    rl[r] = j;
+    rs[r] = rowstart[(long long)k * N + j];
+    rm[2 * r] = rowmask[((long long)k * N + j) * 2];
+    rm[2 * r + 1] = rowmask[((long long)k * N + j) * 2 + 1];
+  }
   __syncthreads();
-  for (int s = tid; s < S; s += 256) plane[jk[s]] = st[s];
+
+  // 1. inverse along k, phase 1, rows with sticks: read the sticks, DFT_N1, twiddle
+  for (int t = tid; t < nr * N2; t += NTH) {
+    const int s = t / N2, j2 = t % N2;
+    const unsigned long long lo = rm[2 * s], hi = rm[2 * s + 1];
+    const C* src = st + rs[s];
+    C u[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+      const int kk = j2 + N2 * n1;
+      u[n1] = mask_bit(lo, hi, kk) ? src[mask_below(lo, hi, kk)] : cmake<C>(0, 0);
+    }
+    small_dft<N1, 1>(u);
+    twiddle_row<N, N1, 1>(u, j2, tw);
+    C* row = plane + rl[s] * N;
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) row[j2 + N2 * k1] = u[k1];
+  }
   __syncthreads();
-  fft2<N, 1, T2, true>(plane, 1, N, nr, rl, tw, tid, 256);       // along k, rows with sticks
-  fft2<N, 1, T2, false>(plane, N, 1, N, nullptr, tw, tid, 256);  // along j, every column
-  const R* v = veff + (long long)i * NP;
-  for (int e = tid; e < NP; e += 256) plane[e] = cscale(plane[e], v[e]);
+  // 2. inverse along k, phase 2 (in place: all reads, barrier, all writes)
+  {
+    C v[TR][N2];
+#pragma unroll
+    for (int q = 0; q < TR; ++q) {
+      const int t = tid + q * NTH;
+      if (t < nr * N1) {
+        const C* row = plane + rl[t / N1] * N + N2 * (t % N1);
+#pragma unroll
+        for (int j2 = 0; j2 < N2; ++j2) v[q][j2] = row[j2];
+        small_dft<N2, 1>(v[q]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TR; ++q) {
+      const int t = tid + q * NTH;
+      if (t < nr * N1) {
+        C* row = plane + rl[t / N1] * N + t % N1;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) row[N1 * k2] = v[q][k2];
+      }
+    }
+  }
   __syncthreads();
-  fft2<N, -1, T2, false>(plane, N, 1, N, nullptr, tw, tid, 256);
-  fft2<N, -1, T2, true>(plane, 1, N, nr, rl, tw, tid, 256);
-  for (int s = tid; s < S; s += 256) st[s] = plane[jk[s]];
+  // 3. inverse along j, phase 1, every column; rows without sticks read as zeros
+  for (int t = tid; t < N * N2; t += NTH) {
+    const int c = t % N, j2 = t / N;
+    C u[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+      const int j = j2 + N2 * n1;
+      u[n1] = mask_bit(live_lo, live_hi, j) ? plane[j * N + c] : cmake<C>(0, 0);
+    }
+    small_dft<N1, 1>(u);
+    twiddle_row<N, N1, 1>(u, j2, tw);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) plane[(j2 + N2 * k1) * N + c] = u[k1];
+  }
+  __syncthreads();
+  // 4. inverse along j phase 2, times V_eff, forward along j phase 1 of the transposed split
+  //    (N = N2 x N1): task (c, k1) holds rows k1 + N1 m, m < N2, from the DFT to the twiddle.
+  {
+    const R* v = veff + (long long)i * N * N;
+    C x[TR][N2];
+#pragma unroll
+    for (int q = 0; q < TR; ++q) {
+      const int t = tid + q * NTH;
+      if (t < N * N1) {
+        const int c = t % N, k1 = t / N;
+#pragma unroll
+        for (int j2 = 0; j2 < N2; ++j2) x[q][j2] = plane[(N2 * k1 + j2) * N + c];
+        small_dft<N2, 1>(x[q]);
+#pragma unroll
+        for (int m = 0; m < N2; ++m) x[q][m] = cscale(x[q][m], v[(k1 + N1 * m) * N + c]);
+        small_dft<N2, -1>(x[q]);
+        twiddle_row<N, N2, -1>(x[q], k1, tw);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TR; ++q) {
+      const int t = tid + q * NTH;
+      if (t < N * N1) {
+        const int c = t % N, k1 = t / N;
+#pragma unroll
+        for (int m = 0; m < N2; ++m) plane[(k1 + N1 * m) * N + c] = x[q][m];
+      }
+    }
+  }
+  __syncthreads();
+  // 5. forward along j, phase 2 of the transposed split: task (c, m) reads rows N1 m + k1,
+  //    k1 < N1, and produces rows m + N2 p, p < N1; only rows with sticks are kept
+  {
+    constexpr int TC = ceil_div(N * N2, NTH);
+    C y[TC][N1];
+#pragma unroll
+    for (int q = 0; q < TC; ++q) {
+      const int t = tid + q * NTH;
+      if (t < N * N2) {
+        const int c = t % N, m = t / N;
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) y[q][k1] = plane[(N1 * m + k1) * N + c];
+        small_dft<N1, -1>(y[q]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TC; ++q) {
+      const int t = tid + q * NTH;
+      if (t < N * N2) {
+        const int c = t % N, m = t / N;
+#pragma unroll
+        for (int p = 0; p < N1; ++p) {
+          const int j = m + N2 * p;
+          if (mask_bit(live_lo, live_hi, j)) plane[j * N + c] = y[q][p];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // 6. forward along k, phase 1, rows with sticks
+  for (int t = tid; t < nr * N2; t += NTH) {
+    const int s = t / N2, j2 = t % N2;
+    C* row = plane + rl[s] * N;
+    C u[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) u[n1] = row[j2 + N2 * n1];
+    small_dft<N1, -1>(u);
+    twiddle_row<N, N1, -1>(u, j2, tw);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) row[j2 + N2 * k1] = u[k1];
+  }
+  __syncthreads();
+  // 7. forward along k, phase 2: outputs at k = k1 + N1 k2 go straight to their sticks
+  for (int t = tid; t < nr * N1; t += NTH) {
+    const int s = t / N1, k1 = t % N1;
+    const C* row = plane + rl[s] * N + N2 * k1;
+    C v[N2];
+#pragma unroll
+    for (int j2 = 0; j2 < N2; ++j2) v[j2] = row[j2];
+    small_dft<N2, -1>(v);
+    const unsigned long long lo = rm[2 * s], hi = rm[2 * s + 1];
+    C* dst = st + rs[s];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const int kk = k1 + N1 * k2;
+      if (mask_bit(lo, hi, kk)) dst[mask_below(lo, hi, kk)] = v[k2];
+    }
+  }
 }
 
-template <int N, int MAXPTS, typename C>
-__global__ __launch_bounds__(512) void k_density_plane_n(const C* __restrict__ sticks,
-                                                         const int* __restrict__ stick_jk,
-                                                         const int* __restrict__ nsticks,
-                                                         const int* __restrict__ rows, const int* __restrict__ nrows,
+// sum_l w_l |u_l(r)|^2 over the bands of one chunk of the list, on one i-plane: the inverse 2D
+// transform of each band's plane as in k_plane_vloc_n (sticks read through the row masks, rows
+// without sticks read as zeros along j), and the second phase along j squares its outputs into
+// FP64 accumulators held by the task that computes them (task (c, k1) always owns rows
+// k1 + N1 m of column c), so the transformed plane is never written back.
+template <int N, typename C>
+__global__ __launch_bounds__(512) void k_density_plane_n(const C* __restrict__ sticks, const int* __restrict__ rows,
+                                                         const int* __restrict__ nrows,
+                                                         const unsigned long long* __restrict__ rowmask,
+                                                         const int* __restrict__ rowstart,
+                                                         const unsigned long long* __restrict__ livemask,
                                                          const int* __restrict__ band_list,
                                                          const double* __restrict__ band_weight, int nlist,
                                                          int chunk, int smax, int nb, int k0,
                                                          const C* __restrict__ tw, double* __restrict__ part) {
+  constexpr int NTH = 512;
   constexpr int NP = N * N;
-  constexpr int T2 = ceil_div(N * Split<N>::N1, 512);
-  static_assert(MAXPTS * 512 >= NP, "density accumulator too small");
+  constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  constexpr int TR = ceil_div(N * N1, NTH);
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   C* plane = reinterpret_cast<C*>(smem_raw);
-  __shared__ int rl[(N + 3) / 4 * 4];  // a multiple of 16 bytes: the dynamic plane after it stays 16-byte aligned
+  __shared__ int rl[(N + 3) / 4 * 4];  // sizes in multiples of 16 bytes keep the dynamic plane 16-byte aligned
+  __shared__ int rs[(N + 3) / 4 * 4];
+  __shared__ unsigned long long rm[2 * N];
   const int tid = threadIdx.x;
   const int i = blockIdx.x;
   const int ch = blockIdx.y;
-  double acc[MAXPTS];
+  double acc[TR][N2];
 #pragma unroll
-  for (int q = 0; q < MAXPTS; ++q) acc[q] = 0.0;
+  for (int q = 0; q < TR; ++q)
+#pragma unroll
+    for (int m = 0; m < N2; ++m) acc[q][m] = 0.0;
   const int l0 = ch * chunk;
   const int l1 = (l0 + chunk < nlist) ? l0 + chunk : nlist;
   for (int l = l0; l < l1; ++l) {
     const int kb = band_list[l];
     const double wgt = band_weight[l];
     const int k = k0 + kb / nb;
-    const int S = nsticks[k];
     const int nr = nrows[k];
-    const int* jk = stick_jk + (long long)k * smax;
+    const unsigned long long live_lo = livemask[2 * k], live_hi = livemask[2 * k + 1];
     const C* st = sticks + ((long long)kb * N + i) * smax;
-    for (int e = tid; e < NP; e += 512) plane[e] = cmake<C>(0, 0);
-    for (int r = tid; r < nr; r += 512) rl[r] = rows[(long long)k * N + r];
+    for (int r = tid; r < nr; r += NTH) {
+      const int j = rows[(long long)k * N + r];
+      rl[r] = j;
+      rs[r] = rowstart[(long long)k * N + j];
+      rm[2 * r] = rowmask[((long long)k * N + j) * 2];
+      rm[2 * r + 1] = rowmask[((long long)k * N + j) * 2 + 1];
+    }
     __syncthreads();
-    for (int s = tid; s < S; s += 512) plane[jk[s]] = st[s];
-    __syncthreads();
-    fft2<N, 1, T2, true>(plane, 1, N, nr, rl, tw, tid, 512);
-    fft2<N, 1, T2, false>(plane, N, 1, N, nullptr, tw, tid, 512);
+    // inverse along k, phase 1 (sticks read through the row masks)
+    for (int t = tid; t < nr * N2; t += NTH) {
+      const int s = t / N2, j2 = t % N2;
+      const unsigned long long lo = rm[2 * s], hi = rm[2 * s + 1];
+      const C* src = st + rs[s];
+      C u[N1];
 #pragma unroll
-    for (int q = 0; q < MAXPTS; ++q) {
-      const int e = tid + q * 512;
-      if (e < NP) {
-        const C u = plane[e];
-        acc[q] += wgt * ((double)u.x * (double)u.x + (double)u.y * (double)u.y);
+      for (int n1 = 0; n1 < N1; ++n1) {
+        const int kk = j2 + N2 * n1;
+        u[n1] = mask_bit(lo, hi, kk) ? src[mask_below(lo, hi, kk)] : cmake<C>(0, 0);
+      }
+      small_dft<N1, 1>(u);
+      twiddle_row<N, N1, 1>(u, j2, tw);
+      C* row = plane + rl[s] * N;
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) row[j2 + N2 * k1] = u[k1];
+    }
+    __syncthreads();
+    // inverse along k, phase 2 (in place)
+    {
+      C v[TR][N2];
+#pragma unroll
+      for (int q = 0; q < TR; ++q) {
+        const int t = tid + q * NTH;
+        if (t < nr * N1) {
+          const C* row = plane + rl[t / N1] * N + N2 * (t % N1);
+#pragma unroll
+          for (int j2 = 0; j2 < N2; ++j2) v[q][j2] = row[j2];
+          small_dft<N2, 1>(v[q]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < TR; ++q) {
+        const int t = tid + q * NTH;
+        if (t < nr * N1) {
+          C* row = plane + rl[t / N1] * N + t % N1;
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) row[N1 * k2] = v[q][k2];
+        }
       }
     }
     __syncthreads();
+    // inverse along j, phase 1 (rows without sticks read as zeros)
+    for (int t = tid; t < N * N2; t += NTH) {
+      const int c = t % N, j2 = t / N;
+      C u[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) {
+        const int j = j2 + N2 * n1;
+        u[n1] = mask_bit(live_lo, live_hi, j) ? plane[j * N + c] : cmake<C>(0, 0);
+      }
+      small_dft<N1, 1>(u);
+      twiddle_row<N, N1, 1>(u, j2, tw);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) plane[(j2 + N2 * k1) * N + c] = u[k1];
+    }
+    __syncthreads();
+    // inverse along j, phase 2, squared into the accumulators
+#pragma unroll
+    for (int q = 0; q < TR; ++q) {
+      const int t = tid + q * NTH;
+      if (t < N * N1) {
+        const int c = t % N, k1 = t / N;
+        C x[N2];
+#pragma unroll
+        for (int j2 = 0; j2 < N2; ++j2) x[j2] = plane[(N2 * k1 + j2) * N + c];
+        small_dft<N2, 1>(x);
+#pragma unroll
+        for (int m = 0; m < N2; ++m) acc[q][m] += wgt * ((double)x[m].x * (double)x[m].x + (double)x[m].y * (double)x[m].y);
+      }
+    }
+    __syncthreads();  // the next band overwrites the plane and the row lists
   }
   double* out = part + ((long long)ch * N + i) * NP;
 #pragma unroll
-  for (int q = 0; q < MAXPTS; ++q) {
-    const int e = tid + q * 512;
-    if (e < NP) out[e] = acc[q];
+  for (int q = 0; q < TR; ++q) {
+    const int t = tid + q * NTH;
+    if (t < N * N1) {
+      const int c = t % N, k1 = t / N;
+#pragma unroll
+      for (int m = 0; m < N2; ++m) out[(k1 + N1 * m) * N + c] = acc[q][m];
+    }
   }
 }
 
@@ -203,21 +479,22 @@ static int hpsi_fast_t(const Plan& p, const void* psi_, void* hpsi_, int nb, int
   const C* tw = prec == kFP64 ? (const C*)p.d_tw64[0] : (const C*)p.d_tw32[0];
   const R* kin = prec == kFP64 ? (const R*)p.d_kin64 : (const R*)p.d_kin32;
   const int nkb = nb * nkk;
-  const dim3 sgrid((p.smax + W - 1) / W, nkb);
+  const int nch = (p.smax + W - 1) / W;
+  const dim3 sgrid(stick_grid(nch, nkb));
   const size_t plds = (size_t)N * N * sizeof(C);
   if (stage == 0 || stage == 1) {
     hipLaunchKernelGGL((k_stick_inverse_n<N, C>), sgrid, dim3(256), 0, s, psi, sticks, p.d_stick_map, p.smax,
-                       p.npw_pad, nb, k0, tw);
+                       p.npw_pad, nb, k0, tw, nch, nkb);
     if (hipGetLastError() != hipSuccess) return 1;
   }
   if (stage == 0 || stage == 2) {
-    hipLaunchKernelGGL((k_plane_vloc_n<N, C, R>), dim3(N, nkb), dim3(256), plds, s, sticks, p.d_stick_jk,
-                       p.d_nsticks, p.d_rows, p.d_nrows, veff, p.smax, nb, k0, tw);
+    hipLaunchKernelGGL((k_plane_vloc_n<N, C, R>), dim3(N, nkb), dim3(256), plds, s, sticks, p.d_rows, p.d_nrows,
+                       p.d_rowmask, p.d_rowstart, p.d_livemask, veff, p.smax, nb, k0, tw);
     if (hipGetLastError() != hipSuccess) return 1;
   }
   if (stage == 0 || stage == 3) {
     hipLaunchKernelGGL((k_stick_forward_n<N, C, R>), sgrid, dim3(256), 0, s, sticks, psi, hpsi, p.d_stick_map, kin,
-                       p.smax, p.npw_pad, nb, k0, tw, (R)(1.0 / ((double)N * N * N)));
+                       p.smax, p.npw_pad, nb, k0, tw, (R)(1.0 / ((double)N * N * N)), nch, nkb);
     if (hipGetLastError() != hipSuccess) return 1;
   }
   return 0;
@@ -229,17 +506,17 @@ static int density_fast_t(const Plan& p, const void* psi_, int nb, int k0, int n
   using C = typename Cplx<R>::T;
   constexpr int prec = sizeof(R) == 8 ? kFP64 : kFP32;
   constexpr int W = stick_width<C>();
-  constexpr int MAXPTS = ceil_div(N * N, 512);
   const C* tw = prec == kFP64 ? (const C*)p.d_tw64[0] : (const C*)p.d_tw32[0];
   const int nkb = nb * nkk;
   const int nchunk = (nlist + chunk - 1) / chunk;
-  hipLaunchKernelGGL((k_stick_inverse_n<N, C>), dim3((p.smax + W - 1) / W, nkb), dim3(256), 0, s,
+  const int nch = (p.smax + W - 1) / W;
+  hipLaunchKernelGGL((k_stick_inverse_n<N, C>), dim3(stick_grid(nch, nkb)), dim3(256), 0, s,
                      static_cast<const C*>(psi_), static_cast<C*>(sticks_), p.d_stick_map, p.smax, p.npw_pad, nb,
-                     k0, tw);
+                     k0, tw, nch, nkb);
   if (hipGetLastError() != hipSuccess) return 1;
-  hipLaunchKernelGGL((k_density_plane_n<N, MAXPTS, C>), dim3(N, nchunk), dim3(512), (size_t)N * N * sizeof(C), s,
-                     static_cast<const C*>(sticks_), p.d_stick_jk, p.d_nsticks, p.d_rows, p.d_nrows, list, weight,
-                     nlist, chunk, p.smax, nb, k0, tw, part);
+  hipLaunchKernelGGL((k_density_plane_n<N, C>), dim3(N, nchunk), dim3(512), (size_t)N * N * sizeof(C), s,
+                     static_cast<const C*>(sticks_), p.d_rows, p.d_nrows, p.d_rowmask, p.d_rowstart, p.d_livemask,
+                     list, weight, nlist, chunk, p.smax, nb, k0, tw, part);
   return hipGetLastError() != hipSuccess ? 1 : 0;
 }
 

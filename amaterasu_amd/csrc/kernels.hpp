@@ -40,11 +40,25 @@ struct Plan {
   int* d_nsticks = nullptr;    // [nk]
   int* d_rows = nullptr;       // [nk][n2] the rows j that hold sticks (first nrows[k] entries)
   int* d_nrows = nullptr;      // [nk]
+  // Per k-point and row j (n3 <= 128 only): which columns k of row j hold sticks, as a
+  // 128-bit mask (two words), and the index of the row's first stick (sticks are sorted by
+  // j * n3 + k, so stick (j, k) = rowstart + number of set bits of the mask below k).
+  unsigned long long* d_rowmask = nullptr;  // [nk][n2][2]
+  int* d_rowstart = nullptr;                // [nk][n2]
+  unsigned long long* d_livemask = nullptr; // [nk][2]: rows j that hold sticks (n2 <= 128)
   bool fast = false;           // cubic grid of a length with compile-time kernels
   double* d_kin64 = nullptr;   // [nk][npw_pad] |k+G|^2/2 (0 on padding)
   float* d_kin32 = nullptr;
   double2* d_tw64[3] = {nullptr, nullptr, nullptr};
   float2* d_tw32[3] = {nullptr, nullptr, nullptr};
+  // Kleinman-Bylander tables the caller attached (amt_plan_set_projectors), per precision:
+  // matrix [nk][nproj][npw_pad] complex, D [nk][nproj][nproj] real, atom blocks [nproj].
+  // Caller-owned device memory: the plan keeps the pointers, not copies.
+  int nproj[2] = {0, 0};
+  const void* kb_matrix[2] = {nullptr, nullptr};
+  const void* kb_dij[2] = {nullptr, nullptr};
+  const int* kb_blk_start[2] = {nullptr, nullptr};
+  const int* kb_blk_len[2] = {nullptr, nullptr};
   // launch geometry
   int stick_w[2] = {0, 0};         // sticks per workgroup, [prec]
   int stick_threads[2] = {0, 0};   // workgroup size of the stick kernels
@@ -76,8 +90,14 @@ int launch_density(const Plan& p, int prec, const void* psi, int nb, int k0, int
 int launch_scatter(const Plan& p, int prec, const void* psi, void* grid, int nb, int k, hipStream_t s);
 int launch_gather(const Plan& p, int prec, const void* grid, void* out, int nb, int k, double scale,
                   hipStream_t s);
-int launch_fft3d(int prec, int n1, int n2, int n3, int batch, void* data, int sign, double scale,
-                 hipStream_t s);
+// Twiddle tables and axis factorisations of one 3-D grid (amt_fft_plan_create).
+struct FftPlan {
+  int n[3] = {0, 0, 0};
+  AxisPlan ap[3];
+  double2* tw64[3] = {nullptr, nullptr, nullptr};
+  float2* tw32[3] = {nullptr, nullptr, nullptr};
+};
+int launch_fft3d(const FftPlan& fp, int prec, int batch, void* data, int sign, double scale, hipStream_t s);
 
 // C = alpha * A op(B) + beta * C, complex, row-major; op = 0: B is K x N, op = 1: B is N x K
 // and conj(B)^T is used. Batched with element strides. `ws` holds split-K partials.

@@ -78,6 +78,7 @@ class DevicePlan:
         self.kinetic = {FP64: torch.from_numpy(kpad).to(self.device)}
         self.kinetic[FP32] = self.kinetic[FP64].to(torch.float32)
         self._kb = {}  # precision -> (matrix [nk][nproj][npw_pad], dij [nk][nproj][nproj], start, length)
+        self._attached = {}  # precision -> the kb tables whose pointers the library plan holds
         self._mat64 = None
         self._proj_src = projectors
         self.nproj = 0 if projectors is None else len(projectors[0].labels)
@@ -102,10 +103,25 @@ class DevicePlan:
 
     # ---------------------------------------------------------------- projectors
     def set_projectors(self, projectors) -> None:
+        for prec in list(self._attached):
+            check(lib().amt_plan_set_projectors(self._handle, prec, 0, None, None, None, None), "plan_set_projectors")
+        self._attached = {}
         self._proj_src = projectors
         self._kb = {}
         self._mat64 = None
         self.nproj = 0 if projectors is None else len(projectors[0].labels)
+
+    def _attach(self, prec: int):
+        """Hand this precision's projector tables to the library plan (amt_plan_set_projectors);
+        the tensors stay cached here, so the pointers the plan holds stay valid."""
+        kb = self.kb_tables(prec)
+        if kb is None:
+            return
+        if self._attached.get(prec) is not kb:
+            mat, dij, start, length = kb
+            check(lib().amt_plan_set_projectors(self._handle, prec, self.nproj, mat.data_ptr(), dij.data_ptr(),
+                                                start.data_ptr(), length.data_ptr()), "plan_set_projectors")
+            self._attached[prec] = kb
 
     def _matrix64(self) -> torch.Tensor:
         if self._mat64 is None:
@@ -174,6 +190,14 @@ class DevicePlan:
             if tuple(h.shape) != tuple(p3.shape) or h.dtype != p3.dtype or not h.is_contiguous():
                 raise AmtError("hpsi: out must be a contiguous block shaped like psi (zero padding)")
         if nb == 0:
+            return h[0] if single else h
+        if self.plane_ok[prec] and coupling is None:
+            # the whole operator in one library call (local part + projector GEMMs)
+            self._attach(prec)
+            nbytes = lib().amt_hpsi_workspace(self._handle, prec, nb, nkk)
+            ws = ops.workspace(nbytes, f"hpsi{id(self)}")
+            check(lib().amt_hpsi(self._handle, prec, p3.data_ptr(), h.data_ptr(), nb, k0, nkk, v.data_ptr(),
+                                 ws.data_ptr(), nbytes, ops.stream_handle()), "hpsi")
             return h[0] if single else h
         if self.plane_ok[prec]:
             st = self.sticks(prec, nkk * nb)

@@ -6,6 +6,8 @@ raises otherwise; nothing falls back to a CPU implementation.
 
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
 from . import autotune
@@ -114,13 +116,46 @@ def zgemm(a: torch.Tensor, b: torch.Tensor, op: int, *, alpha: float = 1.0, beta
     return out
 
 
+class _FFTPlan:
+    """amt_fft_plan_create / amt_fft_plan_destroy for one grid shape on one device."""
+
+    def __init__(self, shape):
+        handle = ctypes.c_void_p()
+        check(lib().amt_fft_plan_create(ctypes.byref(handle), *(int(n) for n in shape)), "fft_plan_create")
+        self.handle = handle
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                lib().amt_fft_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+_FFT_PLANS: dict = {}
+
+
+def fft_plan(shape) -> "_FFTPlan":
+    """The (cached) transform plan of a grid shape on the current device: its twiddle tables are
+    allocated once here, so the transforms themselves allocate nothing."""
+    key = (tuple(int(n) for n in shape), torch.cuda.current_device())
+    plan = _FFT_PLANS.get(key)
+    if plan is None:
+        plan = _FFTPlan(key[0])
+        _FFT_PLANS[key] = plan
+    return plan
+
+
 def fft3d(x: torch.Tensor, sign: int, scale: float = 1.0, *, inplace: bool = False) -> torch.Tensor:
     """Batched 3D transform over the last three axes (sign +1: e^{+iGr}, -1: e^{-iGr})."""
     _require(x, "x")
     y = x if inplace else x.clone()
     n1, n2, n3 = y.shape[-3:]
     batch = int(y.numel() // (n1 * n2 * n3))
-    check(lib().amt_fft3d(precision_of(y), n1, n2, n3, batch, y.data_ptr(), int(sign), float(scale),
+    plan = fft_plan((n1, n2, n3))
+    check(lib().amt_fft3d(plan.handle, precision_of(y), batch, y.data_ptr(), int(sign), float(scale),
                           stream_handle()), "fft3d")
     return y
 

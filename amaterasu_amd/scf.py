@@ -477,6 +477,7 @@ def run_scf(system, settings: SCFSettings, *, density=None, wavefunctions=None, 
         if prec == "fp32":
             tol = max(tol, settings.fp32_tol_floor)
         steps = 40 if it == 1 else 20
+        steps_before = dav_steps
         for sp in range(nspin):
             if settings.batched_k and nk > 1:
                 kin_b = plan.kinetic[1 if cdt == torch.complex128 else 0]
@@ -536,7 +537,7 @@ def run_scf(system, settings: SCFSettings, *, density=None, wavefunctions=None, 
         hist.append((e, resid, identity))
         if log:
             log(f"{it:4d} {e * HARTREE_IN_EV:18.9f} {delta * HARTREE_IN_EV / natoms:14.2e} {resid:13.3e} "
-                f"{identity * HARTREE_IN_EV:12.2e}")
+                f"{identity * HARTREE_IN_EV:12.2e}  {prec} tol {tol:.1e} davidson {dav_steps - steps_before}")
         if it > 1 and abs(delta) < ethr and resid < settings.tol_density:
             conv = True
             break

@@ -142,6 +142,8 @@ def main(argv=None) -> int:
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "mixed"])
     ap.add_argument("--learned", action="store_true", help="start from the trained network's density")
     ap.add_argument("--max-steps", type=int, default=60)
+    ap.add_argument("--fp32-tol-floor", type=float, default=None, help="eigensolver tolerance floor of FP32 blocks")
+    ap.add_argument("--fp64-below", type=float, default=None, help="mixed: density residual of the switch to FP64")
     a = ap.parse_args(argv)
     import torch
 
@@ -154,8 +156,13 @@ def main(argv=None) -> int:
         system, smearing = si214_kgrid_system(), {"scheme": "fermi-dirac", "smearing": 0.005}
     t_setup = time.perf_counter()
     rho = learned_density(system, DensityModel.load()) if a.learned else system.initial_density()
+    extra = {}
+    if a.fp32_tol_floor is not None:
+        extra["fp32_tol_floor"] = a.fp32_tol_floor
+    if a.fp64_below is not None:
+        extra["fp64_below"] = a.fp64_below
     settings = SCFSettings(tol_energy=1e-6 * HARTREE_IN_EV, tol_density=1e-6, max_steps=a.max_steps,
-                           precision=a.precision, **smearing)
+                           precision=a.precision, **smearing, **extra)
     torch.cuda.synchronize()
     t = t_setup
     print(f"system ready: {len(system.symbols)} atoms, grid {system.grid.shape}, "

@@ -11,7 +11,8 @@
  *     [k][band][npw_pad]. Grids are C-ordered, flat = (i*n2 + j)*n3 + k.
  *   - `stream` is a hipStream_t (0 = the null stream); every call is asynchronous on it.
  *   - Workspaces (sticks, density partials, GEMM split-K partials) are passed in by the
- *     caller; compute entry points allocate nothing. Only amt_plan_create allocates.
+ *     caller; compute entry points allocate nothing. Only amt_plan_create and
+ *     amt_fft_plan_create allocate (amt_plan_set_projectors keeps caller pointers).
  *
  * Reference interfaces (amaterasu 0.1.0a2, paths relative to its source tree) that the
  * entry points replace are cited per function. The Python binding a maintainer adds to
@@ -67,6 +68,29 @@ int amt_hpsi_local(const amt_plan_t* plan, int precision, const void* psi, void*
                    const void* veff, void* sticks, void* stream);
 
 /*
+ * Attach the Kleinman-Bylander tables of KBProjectors (pseudo/kb.py:199-260, built by
+ * build_projectors kb.py:260-343) to the plan, for amt_hpsi at `precision`:
+ *   matrix   : [nk][nproj][npw_pad] complex (beta, zero on padded columns), device;
+ *   dij      : [nk][nproj][nproj] real D, block-diagonal over atoms, device;
+ *   blk_start, blk_len : [nproj] int, the atom block (first row, length) of each row, device.
+ * The plan keeps the pointers (the memory stays the caller's and must outlive its use by
+ * amt_hpsi); nothing is copied or allocated. nproj = 0 detaches the tables.
+ */
+int amt_plan_set_projectors(amt_plan_t* plan, int precision, int nproj, const void* matrix, const void* dij,
+                            const int* blk_start, const int* blk_len);
+/* Bytes of the workspace amt_hpsi needs for nb bands at each of nk k-points. */
+size_t amt_hpsi_workspace(const amt_plan_t* plan, int precision, int nb, int nk);
+/*
+ * The whole operator of Hamiltonian.apply (hamiltonian/operator.py:60-69) in one call:
+ *   hpsi = T_G psi + gather(FFT(V_eff . IFFT(scatter psi))) + (psi beta^H) D beta
+ * for nb bands at each of the k-points k0..k0+nk-1 ([nk][nb][npw_pad], the projector term of
+ * KBProjectors.apply, pseudo/kb.py:216-230, with the tables of amt_plan_set_projectors; no
+ * projector term when none are attached). `ws` holds amt_hpsi_workspace(...) bytes.
+ */
+int amt_hpsi(const amt_plan_t* plan, int precision, const void* psi, void* hpsi, int nb, int k0, int nk,
+             const void* veff, void* ws, size_t ws_bytes, void* stream);
+
+/*
  * amt_hpsi_local without a workspace argument and without any device allocation. No
  * counterpart in the reference: for callers that cannot allocate (code under stream or
  * graph capture, tight serving loops, harnesses that forbid hipMalloc). The stick
@@ -105,12 +129,18 @@ int amt_scatter(const amt_plan_t* plan, int precision, const void* psi, void* gr
 int amt_gather(const amt_plan_t* plan, int precision, const void* grid, void* out, int nb, int k, double scale,
                void* stream);
 /*
- * In-place batched 3D transform over the last three axes, times `scale`:
+ * In-place batched 3D transform over the last three axes of an n1 x n2 x n3 grid, times `scale`:
  *   sign = +1: u(r) = sum_G c_G e^{+iG.r}      (FFTGrid.to_real,      basis/fft.py:219-226, scale 1)
  *   sign = -1: c_G = sum_r u(r) e^{-iG.r}      (FFTGrid.to_reciprocal, basis/fft.py:228-235, scale 1/N)
- * Lengths must be 2,3,5,7,11-smooth (the grid rule of basis/fft.py:72-81).
+ * Lengths must be 2,3,5,7,11-smooth (the grid rule of basis/fft.py:72-81). amt_fft_plan_create
+ * factorises the lengths and allocates the twiddle tables of both precisions (the only call
+ * here that allocates); amt_fft3d then allocates nothing and is asynchronous on `stream`.
+ * A plan may be used from several streams at once (it is read-only after creation).
  */
-int amt_fft3d(int precision, int n1, int n2, int n3, int batch, void* data, int sign, double scale, void* stream);
+typedef struct amt_fft_plan amt_fft_plan_t;
+int amt_fft_plan_create(amt_fft_plan_t** out, int n1, int n2, int n3);
+int amt_fft_plan_destroy(amt_fft_plan_t* plan);
+int amt_fft3d(const amt_fft_plan_t* plan, int precision, int batch, void* data, int sign, double scale, void* stream);
 
 /*
  * C = alpha A op(B) + beta C on the matrix cores (complex, row-major, real alpha/beta):

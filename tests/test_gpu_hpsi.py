@@ -201,3 +201,68 @@ def test_full_grid_route_matches_the_plane_kernels(si2_system, precision):
     tol = 1e-12 if precision == "fp64" else 2e-6
     assert (torch.abs(got - ref).max() / torch.abs(ref).max()).item() < tol
     assert (torch.abs(rho_b - rho_a).max() / rho_a.abs().max()).item() < tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_full_operator_entry_point_matches_the_call_sequence(si_pseudo, precision):
+    """amt_hpsi (T + V_loc + V_NL in one call, projector tables attached to the plan with
+    amt_plan_set_projectors) against the explicit sequence amt_hpsi_local, amt_zgemm (NC),
+    amt_kb_dij, amt_zgemm (NN) through the C ABI on 3 ragged k-points of the Si8 batch:
+    bit-identical, and a plan without tables gives the local part alone."""
+    import torch
+
+    from amaterasu_amd import ops
+    from amaterasu_amd._lib import check, lib
+    from amaterasu_amd.device import DevicePlan
+    from amaterasu_amd.system import build_system, load_dataset
+    from amaterasu_amd.workloads import seeded_coefficients, si8_kbatch, smooth_potential
+
+    spec = si8_kbatch(nk=3, nbands=7)
+    s = build_system(spec.cell, spec.positions, spec.symbols, {"Si": load_dataset()}, spec.ecut_rho, spec.kpoints,
+                     spec.ecuts)
+    plan = DevicePlan(s.grid.shape, s.wave_spheres, s.projectors)
+    prec = 1 if precision == "fp64" else 0
+    cdt = torch.complex128 if prec else torch.complex64
+    psi = torch.zeros((3, 7, plan.npw_pad), dtype=cdt, device="cuda")
+    for k, sp in enumerate(s.wave_spheres):
+        psi[k, :, : len(sp.g2)] = torch.from_numpy(seeded_coefficients(sp.g2, 7, seed=40 + k)).to("cuda", cdt)
+    v = torch.from_numpy(smooth_potential(s.grid.shape, seed=4)).to("cuda", ops.real_dtype(prec))
+    mat, dij, start, length = plan.kb_tables(prec)
+    stream = ops.stream_handle()
+    # explicit sequence
+    ref = torch.zeros_like(psi)
+    st = plan.sticks(prec, 21)
+    check(lib().amt_hpsi_local(plan.handle, prec, psi.data_ptr(), ref.data_ptr(), 7, 0, 3, v.data_ptr(),
+                               st.data_ptr(), stream), "hpsi_local")
+    local = ref.clone()
+    proj = ops.zgemm(psi, mat, 1)
+    pd = torch.empty_like(proj)
+    check(lib().amt_kb_dij(prec, proj.data_ptr(), dij.data_ptr(), start.data_ptr(), length.data_ptr(), pd.data_ptr(),
+                           7, plan.nproj, 7 * plan.nproj, plan.nproj * plan.nproj, 7 * plan.nproj, 3, stream), "kb_dij")
+    ops.zgemm(pd, mat, 0, beta=1.0, out=ref)
+    # one call
+    check(lib().amt_plan_set_projectors(plan.handle, prec, plan.nproj, mat.data_ptr(), dij.data_ptr(),
+                                        start.data_ptr(), length.data_ptr()), "plan_set_projectors")
+    nbytes = lib().amt_hpsi_workspace(plan.handle, prec, 7, 3)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    got = torch.zeros_like(psi)
+    check(lib().amt_hpsi(plan.handle, prec, psi.data_ptr(), got.data_ptr(), 7, 0, 3, v.data_ptr(), ws.data_ptr(),
+                         nbytes, stream), "hpsi")
+    torch.cuda.synchronize()
+    assert torch.equal(got, ref)
+    # a single k-point from the middle of the batch
+    one = torch.zeros_like(psi[1:2])
+    check(lib().amt_hpsi(plan.handle, prec, psi[1:2].data_ptr(), one.data_ptr(), 7, 1, 1, v.data_ptr(),
+                         ws.data_ptr(), nbytes, stream), "hpsi")
+    torch.cuda.synchronize()
+    assert (torch.abs(one - ref[1:2]).max() / torch.abs(ref).max()).item() < (1e-14 if prec else 1e-6)
+    # detached tables: the local part alone
+    check(lib().amt_plan_set_projectors(plan.handle, prec, 0, None, None, None, None), "plan_set_projectors")
+    bare = torch.zeros_like(psi)
+    check(lib().amt_hpsi(plan.handle, prec, psi.data_ptr(), bare.data_ptr(), 7, 0, 3, v.data_ptr(), ws.data_ptr(),
+                         nbytes, stream), "hpsi")
+    torch.cuda.synchronize()
+    assert torch.equal(bare, local)
+    with pytest.raises(Exception, match="workspace smaller"):
+        check(lib().amt_hpsi(plan.handle, prec, psi.data_ptr(), bare.data_ptr(), 7, 0, 3, v.data_ptr(),
+                             ws.data_ptr(), 16, stream), "hpsi")
