@@ -43,12 +43,12 @@ static int upload(T** dst, const std::vector<T>& src, const char* what) {
 static void free_plan_device(Plan& p) {
   void* ptrs[] = {p.d_flatmap,  p.d_stick_jk, p.d_stick_map, p.d_nsticks, p.d_rows,       p.d_nrows,
                   p.d_kin64,    p.d_kin32,    p.d_tw64[0],   p.d_tw64[1],  p.d_tw64[2],    p.d_tw32[0],
-                  p.d_tw32[1],  p.d_tw32[2],  p.d_rowmask,   p.d_rowstart, p.d_livemask};
+                  p.d_tw32[1],  p.d_tw32[2],  p.d_rowmask,   p.d_rowstart, p.d_livemask,   p.d_npw};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   p.d_flatmap = p.d_stick_jk = p.d_stick_map = p.d_nsticks = p.d_rows = p.d_nrows = nullptr;
   p.d_rowmask = p.d_livemask = nullptr;
-  p.d_rowstart = nullptr;
+  p.d_rowstart = p.d_npw = nullptr;
   p.d_kin64 = nullptr;
   p.d_kin32 = nullptr;
   for (int a = 0; a < 3; ++a) {
@@ -251,6 +251,8 @@ int amt_plan_create(amt_plan_t** out, int n1, int n2, int n3, int nk, const int*
     else
       p.plane_threads[prec] = (pt <= 1024 && plds <= lds_max) ? pt : 0;
   }
+  p.plane_fit[0] = p.plane_threads[0];
+  p.plane_fit[1] = p.plane_threads[1];
   if (p.stick_threads[0] > 1024 || p.stick_threads[1] > 1024) {
     set_error("plan_create: axis 0 too long for the stick kernels");
     delete h;
@@ -261,6 +263,7 @@ int amt_plan_create(amt_plan_t** out, int n1, int n2, int n3, int nk, const int*
       !upload(&p.d_stick_map, smap, "plan stick map") || !upload(&p.d_nsticks, nst, "plan nsticks") ||
       !upload(&p.d_rows, rows, "plan rows") || !upload(&p.d_nrows, nrows, "plan nrows") ||
       !upload(&p.d_kin64, kin64, "plan kinetic") || !upload(&p.d_kin32, kin32, "plan kinetic32") ||
+      !upload(&p.d_npw, p.npw, "plan npw") ||
       (!rowmask.empty() && (!upload(&p.d_rowmask, rowmask, "plan row masks") ||
                             !upload(&p.d_rowstart, rowstart, "plan row starts") ||
                             !upload(&p.d_livemask, livemask, "plan live rows")))) {
@@ -311,9 +314,31 @@ int amt_plan_nsticks(const amt_plan_t* h, int* out) {
   return 0;
 }
 
+// Bytes of the local H.psi / density workspace for `rows` band rows: their sticks, or, where
+// the plane does not fit LDS (full-grid route), whole grids for min(rows, ~1 GiB of grids).
+static size_t local_ws_bytes(const Plan& p, int prec, long long rows) {
+  if (rows <= 0 || (prec != kFP32 && prec != kFP64)) return 0;
+  const size_t elem = prec == kFP64 ? 16 : 8;
+  if (p.plane_threads[prec] > 0) return (size_t)rows * p.n1 * p.smax * elem;
+  const size_t grid = (size_t)p.n1 * p.n2 * p.n3 * elem;
+  long long g = (long long)((size_t(1) << 30) / grid);
+  if (g < 1) g = 1;
+  return (size_t)(rows < g ? rows : g) * grid;
+}
+
 size_t amt_plan_stick_bytes(const amt_plan_t* h, int precision, int nbands) {
   if (!h || nbands <= 0) return 0;
-  return (size_t)nbands * h->p.n1 * h->p.smax * (precision == kFP64 ? 16 : 8);
+  return local_ws_bytes(h->p, precision, nbands);
+}
+
+int amt_plan_set_route(amt_plan_t* h, int precision, int full_grid) {
+  if (!h || (precision != kFP32 && precision != kFP64)) {
+    set_error("plan_set_route: null plan or bad precision");
+    return 1;
+  }
+  Plan& p = h->p;
+  p.plane_threads[precision] = full_grid ? 0 : p.plane_fit[precision];
+  return 0;
 }
 
 static int check_plane(const Plan& p, int prec) {
@@ -336,7 +361,10 @@ int amt_hpsi_local(const amt_plan_t* h, int precision, const void* psi, void* hp
     return 1;
   }
   const Plan& p = h->p;
-  if (check_plane(p, precision)) return 1;
+  if (precision != kFP32 && precision != kFP64) {
+    set_error("precision must be 0 (fp32) or 1 (fp64)");
+    return 1;
+  }
   if (nb <= 0 || nkk <= 0 || k0 < 0 || k0 + nkk > p.nk) {
     set_error("hpsi_local: band/k range outside the plan");
     return 1;
@@ -345,6 +373,9 @@ int amt_hpsi_local(const amt_plan_t* h, int precision, const void* psi, void* hp
     set_error("hpsi_local: more than 65535 bands in one launch");
     return 1;
   }
+  if (p.plane_threads[precision] == 0)
+    return launch_hpsi_full_grid(p, precision, psi, hpsi, nb, k0, nkk, veff, sticks,
+                                 local_ws_bytes(p, precision, (long long)nb * nkk), (hipStream_t)stream);
   return launch_hpsi_local(p, precision, psi, hpsi, nb, k0, nkk, veff, sticks, (hipStream_t)stream);
 }
 
@@ -376,7 +407,7 @@ struct HpsiLayout {
 HpsiLayout hpsi_layout(const Plan& p, int prec, int nb, int nkk) {
   HpsiLayout l;
   const size_t elem = prec == kFP64 ? 16 : 8;
-  l.sticks = align_up((size_t)nb * nkk * p.n1 * p.smax * elem);
+  l.sticks = align_up(local_ws_bytes(p, prec, (long long)nb * nkk));
   const int np = p.nproj[prec];
   if (np > 0) {
     l.proj = align_up((size_t)nkk * nb * np * elem);
@@ -402,7 +433,10 @@ int amt_hpsi(const amt_plan_t* h, int precision, const void* psi, void* hpsi, in
     return 1;
   }
   const Plan& p = h->p;
-  if (check_plane(p, precision)) return 1;
+  if (precision != kFP32 && precision != kFP64) {
+    set_error("precision must be 0 (fp32) or 1 (fp64)");
+    return 1;
+  }
   if (nb <= 0 || nkk <= 0 || k0 < 0 || k0 + nkk > p.nk || (long long)nb * nkk > 65535) {
     set_error("hpsi: band/k range outside the plan");
     return 1;
@@ -414,7 +448,11 @@ int amt_hpsi(const amt_plan_t* h, int precision, const void* psi, void* hpsi, in
   }
   hipStream_t s = (hipStream_t)stream;
   unsigned char* base = static_cast<unsigned char*>(ws);
-  if (launch_hpsi_local(p, precision, psi, hpsi, nb, k0, nkk, veff, base, s)) return 1;
+  if (p.plane_threads[precision] == 0) {
+    if (launch_hpsi_full_grid(p, precision, psi, hpsi, nb, k0, nkk, veff, base, l.sticks, s)) return 1;
+  } else if (launch_hpsi_local(p, precision, psi, hpsi, nb, k0, nkk, veff, base, s)) {
+    return 1;
+  }
   const int np = p.nproj[precision];
   if (np == 0) return 0;
   // V_NL psi = (P D) beta with P = psi beta^H (pseudo/kb.py:216-230)
@@ -484,11 +522,18 @@ int amt_density(const amt_plan_t* h, int precision, const void* psi, int nb, int
     return 1;
   }
   const Plan& p = h->p;
-  if (check_plane(p, precision)) return 1;
+  if (precision != kFP32 && precision != kFP64) {
+    set_error("precision must be 0 (fp32) or 1 (fp64)");
+    return 1;
+  }
   if (nb <= 0 || nkk <= 0 || k0 < 0 || k0 + nkk > p.nk || chunk <= 0 || nlist < 0) {
     set_error("density: band/k range outside the plan");
     return 1;
   }
+  if (p.plane_threads[precision] == 0)
+    return launch_density_full_grid(p, precision, psi, nb, k0, band_list, band_weight, nlist, sticks,
+                                    local_ws_bytes(p, precision, (long long)nb * nkk), part, rho, beta, scale,
+                                    (hipStream_t)stream);
   if (nlist > 0 && (nlist + chunk - 1) / chunk > 65535) {
     set_error("density: too many chunks");
     return 1;

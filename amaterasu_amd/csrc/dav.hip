@@ -26,47 +26,51 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return t;
 }
 
-// One workgroup per band row; rows laid out [row][npw_pad].
+// One workgroup of kRowThreads per band row (rows laid out [row][npw_pad]): with a few hundred
+// rows of 10^4-10^5 plane waves each, 16 waves per row keep enough loads in flight per CU.
+constexpr int kRowThreads = 1024;
+
 template <typename R>
-__global__ __launch_bounds__(256) void k_residual(const typename Cplx<R>::T* __restrict__ hv,
+__global__ __launch_bounds__(kRowThreads) void k_residual(const typename Cplx<R>::T* __restrict__ hv,
                                                   const typename Cplx<R>::T* __restrict__ v,
                                                   const double* __restrict__ eps, typename Cplx<R>::T* __restrict__ res,
                                                   double* __restrict__ norm, int npw) {
   using C = typename Cplx<R>::T;
-  __shared__ double red[8];
+  __shared__ double red[kRowThreads / 64 + 1];
   const long long off = (long long)blockIdx.x * npw;
   const R e = (R)eps[blockIdx.x];
   double acc = 0.0;
-  for (int g = threadIdx.x; g < npw; g += 256) {
+  for (int g = threadIdx.x; g < npw; g += kRowThreads) {
     const C a = hv[off + g], b = v[off + g];
     const C r = cmake<C>(a.x - e * b.x, a.y - e * b.y);
     res[off + g] = r;
     acc += (double)r.x * r.x + (double)r.y * r.y;
   }
-  acc = block_sum<256>(acc, red);
+  acc = block_sum<kRowThreads>(acc, red);
   if (threadIdx.x == 0) norm[blockIdx.x] = sqrt(acc);
 }
 
 // rows[a] selects the source band of output row a.
 template <typename R>
-__global__ __launch_bounds__(256) void k_precondition(const typename Cplx<R>::T* __restrict__ res,
+__global__ __launch_bounds__(kRowThreads) void k_precondition(const typename Cplx<R>::T* __restrict__ res,
                                                       const typename Cplx<R>::T* __restrict__ v,
                                                       const R* __restrict__ kin, const int* __restrict__ rows,
                                                       typename Cplx<R>::T* __restrict__ out, int npw) {
   using C = typename Cplx<R>::T;
-  __shared__ double red[8];
+  __shared__ double red[kRowThreads / 64 + 1];
   const int src = rows[blockIdx.x];
   const long long si = (long long)src * npw, oi = (long long)blockIdx.x * npw;
   double tb = 0.0;
-  for (int g = threadIdx.x; g < npw; g += 256) {
+  for (int g = threadIdx.x; g < npw; g += kRowThreads) {
     const C b = v[si + g];
     tb += ((double)b.x * b.x + (double)b.y * b.y) * (double)kin[g];
   }
-  tb = block_sum<256>(tb, red);
+  tb = block_sum<kRowThreads>(tb, red);
   const double ref = tb > 1e-8 ? tb : 1e-8;
+  const double rinv = 1.0 / ref;
   double nn = 0.0;
-  for (int g = threadIdx.x; g < npw; g += 256) {
-    const double x = (double)kin[g] / ref;
+  for (int g = threadIdx.x; g < npw; g += kRowThreads) {
+    const double x = (double)kin[g] * rinv;
     const double p = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x));
     const double f = p / (p + 16.0 * x * x * x * x);
     const C r = res[si + g];
@@ -74,9 +78,9 @@ __global__ __launch_bounds__(256) void k_precondition(const typename Cplx<R>::T*
     out[oi + g] = c;
     nn += (double)c.x * c.x + (double)c.y * c.y;
   }
-  nn = block_sum<256>(nn, red);
+  nn = block_sum<kRowThreads>(nn, red);
   const double inv = 1.0 / (sqrt(nn) > 1e-300 ? sqrt(nn) : 1e-300);
-  for (int g = threadIdx.x; g < npw; g += 256) {
+  for (int g = threadIdx.x; g < npw; g += kRowThreads) {
     const C c = out[oi + g];
     out[oi + g] = cmake<C>((R)(c.x * inv), (R)(c.y * inv));
   }
@@ -110,10 +114,10 @@ int launch_residual(int prec, const void* hv, const void* v, const double* eps, 
                     int npw, hipStream_t s) {
   if (nrow <= 0) return 0;
   if (prec == kFP64)
-    hipLaunchKernelGGL(k_residual<double>, dim3(nrow), dim3(256), 0, s, (const double2*)hv, (const double2*)v, eps,
+    hipLaunchKernelGGL(k_residual<double>, dim3(nrow), dim3(kRowThreads), 0, s, (const double2*)hv, (const double2*)v, eps,
                        (double2*)res, norm, npw);
   else
-    hipLaunchKernelGGL(k_residual<float>, dim3(nrow), dim3(256), 0, s, (const float2*)hv, (const float2*)v, eps,
+    hipLaunchKernelGGL(k_residual<float>, dim3(nrow), dim3(kRowThreads), 0, s, (const float2*)hv, (const float2*)v, eps,
                        (float2*)res, norm, npw);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -127,10 +131,10 @@ int launch_precondition(int prec, const void* res, const void* v, const void* ki
                         int nout, int npw, hipStream_t s) {
   if (nout <= 0) return 0;
   if (prec == kFP64)
-    hipLaunchKernelGGL(k_precondition<double>, dim3(nout), dim3(256), 0, s, (const double2*)res, (const double2*)v,
+    hipLaunchKernelGGL(k_precondition<double>, dim3(nout), dim3(kRowThreads), 0, s, (const double2*)res, (const double2*)v,
                        (const double*)kin, rows, (double2*)out, npw);
   else
-    hipLaunchKernelGGL(k_precondition<float>, dim3(nout), dim3(256), 0, s, (const float2*)res, (const float2*)v,
+    hipLaunchKernelGGL(k_precondition<float>, dim3(nout), dim3(kRowThreads), 0, s, (const float2*)res, (const float2*)v,
                        (const float*)kin, rows, (float2*)out, npw);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {

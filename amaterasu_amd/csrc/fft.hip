@@ -306,6 +306,61 @@ __global__ void k_density_reduce(const double* __restrict__ part, int nchunk, lo
   }
 }
 
+// --------------------------------------------------------------------------- full-grid route
+// For grids whose (n2 x n3) plane does not fit one workgroup's LDS at a precision (FP64 planes
+// above ~100^2): whole grids through the library's scatter, 3-D transforms and gather, a chunk
+// of band rows at a time in the caller's workspace.
+
+// grids[b][e] *= v[e]
+template <typename C, typename R>
+__global__ void k_grid_vmul(C* __restrict__ grids, const R* __restrict__ v, long long ngrid, int nbc) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ngrid) return;
+  const R ve = v[e];
+  for (int b = 0; b < nbc; ++b) grids[(long long)b * ngrid + e] = cscale(grids[(long long)b * ngrid + e], ve);
+}
+
+// out[b][g] = kin[g] psi[b][g] + scale grid[b][flatmap[g]] for g < npw (padding not written)
+template <typename C, typename R>
+__global__ void k_gather_kinetic(const C* __restrict__ grid, const C* __restrict__ psi, C* __restrict__ out,
+                                 const int* __restrict__ flatmap, const R* __restrict__ kin, int npw, int npw_pad,
+                                 long long ngrid, R scale) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (g >= npw) return;
+  const C x = psi[(long long)b * npw_pad + g];
+  const C y = grid[(long long)b * ngrid + flatmap[g]];
+  out[(long long)b * npw_pad + g] = cmake<C>(kin[g] * x.x + scale * y.x, kin[g] * x.y + scale * y.y);
+}
+
+// grids[c][flatmap_k[g]] = psi[row][g] for the listed rows list[c0 + c] (k = k0 + row / nb);
+// grids zeroed beforehand
+template <typename C>
+__global__ void k_scatter_list(const C* __restrict__ psi, C* __restrict__ grids, const int* __restrict__ flatmap,
+                               const int* __restrict__ npw, const int* __restrict__ list, int c0, int nb, int k0,
+                               int npw_pad, long long ngrid) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  const int row = list[c0 + c];
+  const int k = k0 + row / nb;
+  if (g >= npw[k]) return;
+  grids[(long long)c * ngrid + flatmap[(long long)k * npw_pad + g]] = psi[(long long)row * npw_pad + g];
+}
+
+// acc[e] += sum_c w[c0 + c] |grids[c][e]|^2 (FP64)
+template <typename C>
+__global__ void k_grid_density(const C* __restrict__ grids, const double* __restrict__ weight, int c0, int nbc,
+                               long long ngrid, double* __restrict__ acc) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ngrid) return;
+  double s = acc[e];
+  for (int c = 0; c < nbc; ++c) {
+    const C u = grids[(long long)c * ngrid + e];
+    s += weight[c0 + c] * ((double)u.x * (double)u.x + (double)u.y * (double)u.y);
+  }
+  acc[e] = s;
+}
+
 // --------------------------------------------------------------------------- launchers
 static int check(hipError_t e, const char* what) {
   if (e != hipSuccess) {
@@ -491,6 +546,119 @@ static int fft3d_t(const FftPlan& fp, int batch, void* data_, int sign, double s
     if (check(hipGetLastError(), "axis fft launch")) return 1;
   }
   return 0;
+}
+
+// The plan's own grid tables as a transform plan (no allocation).
+static FftPlan grid_fft(const Plan& p) {
+  FftPlan f;
+  f.n[0] = p.n1;
+  f.n[1] = p.n2;
+  f.n[2] = p.n3;
+  for (int a = 0; a < 3; ++a) {
+    f.ap[a] = p.ap[a];
+    f.tw64[a] = p.d_tw64[a];
+    f.tw32[a] = p.d_tw32[a];
+  }
+  return f;
+}
+
+long long full_grid_rows(const Plan& p, int prec, size_t ws_bytes) {
+  const long long gb = (long long)p.n1 * p.n2 * p.n3 * (prec == kFP64 ? 16 : 8);
+  return gb > 0 ? (long long)(ws_bytes / (size_t)gb) : 0;
+}
+
+template <typename R>
+static int hpsi_full_grid_t(const Plan& p, const void* psi_, void* hpsi_, int nb, int k0, int nkk, const void* veff_,
+                            void* ws, size_t ws_bytes, hipStream_t s) {
+  using C = typename Cplx<R>::T;
+  constexpr int prec = sizeof(R) == 8 ? kFP64 : kFP32;
+  const long long ngrid = (long long)p.n1 * p.n2 * p.n3;
+  const long long cap = full_grid_rows(p, prec, ws_bytes);
+  if (cap < 1) {
+    set_error("hpsi (full-grid route): the workspace holds no whole grid");
+    return 1;
+  }
+  const FftPlan f = grid_fft(p);
+  const C* psi = static_cast<const C*>(psi_);
+  C* hpsi = static_cast<C*>(hpsi_);
+  C* grids = static_cast<C*>(ws);
+  const R* veff = static_cast<const R*>(veff_);
+  const R* kin = prec == kFP64 ? (const R*)p.d_kin64 : (const R*)p.d_kin32;
+  for (int kk = 0; kk < nkk; ++kk) {
+    const int k = k0 + kk;
+    for (int b0 = 0; b0 < nb; b0 += (int)cap) {
+      const int nbc = (int)(nb - b0 < cap ? nb - b0 : cap);
+      const C* in = psi + ((long long)kk * nb + b0) * p.npw_pad;
+      C* out = hpsi + ((long long)kk * nb + b0) * p.npw_pad;
+      const long long tot = ngrid * nbc;
+      hipLaunchKernelGGL((k_fill_zero<C>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, grids, tot);
+      hipLaunchKernelGGL((k_scatter<C>), dim3((p.npw[k] + 255) / 256, nbc), dim3(256), 0, s, in, grids,
+                         p.d_flatmap + (long long)k * p.npw_pad, p.npw[k], p.npw_pad, ngrid);
+      if (check(hipGetLastError(), "full-grid scatter launch")) return 1;
+      if (launch_fft3d(f, prec, nbc, grids, +1, 1.0, s)) return 1;
+      hipLaunchKernelGGL((k_grid_vmul<C, R>), dim3((unsigned)((ngrid + 255) / 256)), dim3(256), 0, s, grids, veff,
+                         ngrid, nbc);
+      if (check(hipGetLastError(), "full-grid V_eff launch")) return 1;
+      if (launch_fft3d(f, prec, nbc, grids, -1, 1.0, s)) return 1;
+      hipLaunchKernelGGL((k_gather_kinetic<C, R>), dim3((p.npw[k] + 255) / 256, nbc), dim3(256), 0, s, grids, in,
+                         out, p.d_flatmap + (long long)k * p.npw_pad, kin + (long long)k * p.npw_pad, p.npw[k],
+                         p.npw_pad, ngrid, (R)(1.0 / (double)ngrid));
+      if (check(hipGetLastError(), "full-grid gather launch")) return 1;
+    }
+  }
+  return 0;
+}
+
+int launch_hpsi_full_grid(const Plan& p, int prec, const void* psi, void* hpsi, int nb, int k0, int nkk,
+                          const void* veff, void* ws, size_t ws_bytes, hipStream_t s) {
+  return prec == kFP64 ? hpsi_full_grid_t<double>(p, psi, hpsi, nb, k0, nkk, veff, ws, ws_bytes, s)
+                       : hpsi_full_grid_t<float>(p, psi, hpsi, nb, k0, nkk, veff, ws, ws_bytes, s);
+}
+
+// rho = beta rho + scale sum_l w_l |u_l|^2 through whole grids; `acc` is one FP64 grid.
+template <typename R>
+static int density_full_grid_t(const Plan& p, const void* psi_, int nb, int k0, const int* list,
+                               const double* weight, int nlist, void* ws, size_t ws_bytes, double* acc, double* rho,
+                               double beta, double scale, hipStream_t s) {
+  using C = typename Cplx<R>::T;
+  constexpr int prec = sizeof(R) == 8 ? kFP64 : kFP32;
+  const long long ngrid = (long long)p.n1 * p.n2 * p.n3;
+  const unsigned gb = (unsigned)((ngrid + 255) / 256);
+  if (nlist == 0) {  // rho = beta rho
+    hipLaunchKernelGGL(k_density_reduce, dim3(gb), dim3(256), 0, s, (const double*)nullptr, 0, ngrid, rho, beta,
+                       scale);
+    return check(hipGetLastError(), "density reduce launch");
+  }
+  const long long cap = full_grid_rows(p, prec, ws_bytes);
+  if (cap < 1) {
+    set_error("density (full-grid route): the workspace holds no whole grid");
+    return 1;
+  }
+  const FftPlan f = grid_fft(p);
+  C* grids = static_cast<C*>(ws);
+  hipLaunchKernelGGL(k_density_reduce, dim3(gb), dim3(256), 0, s, (const double*)nullptr, 0, ngrid, acc, 0.0, 0.0);
+  if (check(hipGetLastError(), "full-grid density clear launch")) return 1;
+  for (int c0 = 0; c0 < nlist; c0 += (int)cap) {
+    const int nbc = (int)(nlist - c0 < cap ? nlist - c0 : cap);
+    const long long tot = ngrid * nbc;
+    hipLaunchKernelGGL((k_fill_zero<C>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, grids, tot);
+    hipLaunchKernelGGL((k_scatter_list<C>), dim3((p.npw_pad + 255) / 256, nbc), dim3(256), 0, s,
+                       static_cast<const C*>(psi_), grids, p.d_flatmap, p.d_npw, list, c0, nb, k0, p.npw_pad, ngrid);
+    if (check(hipGetLastError(), "full-grid scatter launch")) return 1;
+    if (launch_fft3d(f, prec, nbc, grids, +1, 1.0, s)) return 1;
+    hipLaunchKernelGGL((k_grid_density<C>), dim3(gb), dim3(256), 0, s, grids, weight, c0, nbc, ngrid, acc);
+    if (check(hipGetLastError(), "full-grid density launch")) return 1;
+  }
+  hipLaunchKernelGGL(k_density_reduce, dim3(gb), dim3(256), 0, s, acc, 1, ngrid, rho, beta, scale);
+  return check(hipGetLastError(), "full-grid density reduce launch");
+}
+
+int launch_density_full_grid(const Plan& p, int prec, const void* psi, int nb, int k0, const int* list,
+                             const double* weight, int nlist, void* ws, size_t ws_bytes, double* acc, double* rho,
+                             double beta, double scale, hipStream_t s) {
+  return prec == kFP64
+             ? density_full_grid_t<double>(p, psi, nb, k0, list, weight, nlist, ws, ws_bytes, acc, rho, beta, scale, s)
+             : density_full_grid_t<float>(p, psi, nb, k0, list, weight, nlist, ws, ws_bytes, acc, rho, beta, scale, s);
 }
 
 int launch_fft3d(const FftPlan& fp, int prec, int batch, void* data, int sign, double scale, hipStream_t s) {

@@ -14,8 +14,10 @@
 // D[4 (l / 16) + q][l % 16] for FP32 and D[l / 16 + 4 q][l % 16] for FP64.
 //
 // Tiling: a 256-thread workgroup (4 wavefronts) computes a 64 x 64 complex tile; each
-// wavefront a 32 x 32 quarter as 2 x 2 MFMA blocks. K advances 16 at a time through
-// LDS, stored k-major so fragment reads are unit-stride across lanes. Long-K /
+// wavefront a 32 x 32 quarter as 2 x 2 MFMA blocks. K advances 16 at a time through two
+// LDS buffers, stored k-major so fragment reads are unit-stride across lanes: the next K
+// tile's global loads are issued before this tile's MFMAs and stored into the other buffer
+// after them, so one barrier per tile suffices and the load latency hides under the MFMAs. Long-K /
 // small-output products (projections onto a few hundred bands) are split along K
 // into enough workgroups to fill 256 CUs, with a deterministic second-pass reduction.
 //
@@ -76,8 +78,8 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
   using acc_t = typename Mfma<R>::acc_t;
   constexpr int TBM = BM * TM, TBN = BN * TM, WB = 2 * TM;  // tile, MFMA blocks per wave side
   constexpr int KQ = 4 * TM;                                 // consecutive k per thread in a row load
-  __shared__ C As[BK][TBM + 1];
-  __shared__ C Bs[BK][TBN + 1];
+  __shared__ C As[2][BK][TBM + 1];  // double-buffered K tiles
+  __shared__ C Bs[2][BK][TBN + 1];
   // XCD-aware order: workgroups are handed to the 8 XCDs round-robin by linear id, so
   // logical tile t = (id % 8) * (T8 / 8) + id / 8 gives each XCD a contiguous run of tiles
   // that share an operand panel in its L2: the N-tiles of one M-tile and K-chunk (A panel),
@@ -134,15 +136,17 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
           sumi[a][b][q] = 0.0;
         }
   }
-  int tiles = 0;
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+  // Global -> register loads of one K tile (bounds-checked, conj(B) for NC), and the store of
+  // those registers into an LDS buffer: A k-major As[k][row]; B k-major Bs[k][col].
+  C ra[KQ], rb[KQ];
+  auto gload = [&](int k0) {
     {
       const int row = tid / (BK / KQ), kq = (tid % (BK / KQ)) * KQ;
       const int gm = m0 + row;
 #pragma unroll
       for (int u = 0; u < KQ; ++u) {
         const int gk = k0 + kq + u;
-        As[kq + u][row] = (gm < M && gk < kend) ? A[(long long)gm * lda + gk] : cmake<C>(0, 0);
+        ra[u] = (gm < M && gk < kend) ? A[(long long)gm * lda + gk] : cmake<C>(0, 0);
       }
     }
     if (OP == 0) {
@@ -151,7 +155,7 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
 #pragma unroll
       for (int u = 0; u < KQ; ++u) {
         const int gn = n0 + c0 + u;
-        Bs[kk][c0 + u] = (gk < kend && gn < N) ? B[(long long)gk * ldb + gn] : cmake<C>(0, 0);
+        rb[u] = (gk < kend && gn < N) ? B[(long long)gk * ldb + gn] : cmake<C>(0, 0);
       }
     } else {
       const int col = tid / (BK / KQ), kq = (tid % (BK / KQ)) * KQ;
@@ -161,29 +165,62 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
         const int gk = k0 + kq + u;
         C v = (gn < N && gk < kend) ? B[(long long)gn * ldb + gk] : cmake<C>(0, 0);
         v.y = -v.y;
-        Bs[kq + u][col] = v;
+        rb[u] = v;
       }
     }
-    __syncthreads();
+  };
+  auto sstore = [&](int buf) {
+    {
+      const int row = tid / (BK / KQ), kq = (tid % (BK / KQ)) * KQ;
+#pragma unroll
+      for (int u = 0; u < KQ; ++u) As[buf][kq + u][row] = ra[u];
+    }
+    if (OP == 0) {
+      const int kk = tid >> 4, c0 = (tid & 15) * KQ;
+#pragma unroll
+      for (int u = 0; u < KQ; ++u) Bs[buf][kk][c0 + u] = rb[u];
+    } else {
+      const int col = tid / (BK / KQ), kq = (tid % (BK / KQ)) * KQ;
+#pragma unroll
+      for (int u = 0; u < KQ; ++u) Bs[buf][kq + u][col] = rb[u];
+    }
+  };
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4);
       C a[WB], b[WB];
 #pragma unroll
-      for (int rb = 0; rb < WB; ++rb) a[rb] = As[kr][wr + rb * 16 + (lane & 15)];
+      for (int rb2 = 0; rb2 < WB; ++rb2) a[rb2] = As[buf][kr][wr + rb2 * 16 + (lane & 15)];
 #pragma unroll
-      for (int cb = 0; cb < WB; ++cb) b[cb] = Bs[kr][wc + cb * 16 + (lane & 15)];
+      for (int cb = 0; cb < WB; ++cb) b[cb] = Bs[buf][kr][wc + cb * 16 + (lane & 15)];
 #pragma unroll
-      for (int rb = 0; rb < WB; ++rb)
+      for (int rb2 = 0; rb2 < WB; ++rb2)
 #pragma unroll
         for (int cb = 0; cb < WB; ++cb) {
-          accr[rb][cb] = Mfma<R>::mma(a[rb].x, b[cb].x, accr[rb][cb]);
-          accr[rb][cb] = Mfma<R>::mma(-a[rb].y, b[cb].y, accr[rb][cb]);
-          acci[rb][cb] = Mfma<R>::mma(a[rb].x, b[cb].y, acci[rb][cb]);
-          acci[rb][cb] = Mfma<R>::mma(a[rb].y, b[cb].x, acci[rb][cb]);
+          accr[rb2][cb] = Mfma<R>::mma(a[rb2].x, b[cb].x, accr[rb2][cb]);
+          accr[rb2][cb] = Mfma<R>::mma(-a[rb2].y, b[cb].y, accr[rb2][cb]);
+          acci[rb2][cb] = Mfma<R>::mma(a[rb2].x, b[cb].y, acci[rb2][cb]);
+          acci[rb2][cb] = Mfma<R>::mma(a[rb2].y, b[cb].x, acci[rb2][cb]);
         }
     }
+  };
+  int tiles = 0;
+  // double-buffered: the next tile's global loads are in flight while this tile's MFMAs run,
+  // and one barrier per tile separates the store of the next buffer from the reads of this one
+  if (kbeg < kend) {
+    gload(kbeg);
+    sstore(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    const bool more = k0 + BK < kend;
+    if (more) gload(k0 + BK);
+    compute(buf);
+    if (more) sstore(buf ^ 1);
     __syncthreads();
+    buf ^= 1;
     if constexpr (kTwoLevel) {
       if (++tiles == kFlushTiles) {
         flush();
@@ -352,9 +389,10 @@ static void launch_tile(int op, dim3 grid, hipStream_t s, int m, int n, int k, i
     }
   }
   // 128 x 128 tiles are planned for NC products only
-  hipLaunchKernelGGL((k_zgemm<R, 1, kTwoLevel, TM>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a, lda,
-                     sa, (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, 0);
+  hipLaunchKernelGGL((k_zgemm<R, 1, kTwoLevel, TM>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a,
+                     lda, sa, (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, 0);
 }
+
 
 template <typename R>
 static int zgemm_t(int op, int m, int n, int k, double alpha, const void* a, long long lda, long long sa,

@@ -40,6 +40,7 @@ struct Plan {
   int* d_nsticks = nullptr;    // [nk]
   int* d_rows = nullptr;       // [nk][n2] the rows j that hold sticks (first nrows[k] entries)
   int* d_nrows = nullptr;      // [nk]
+  int* d_npw = nullptr;        // [nk]
   // Per k-point and row j (n3 <= 128 only): which columns k of row j hold sticks, as a
   // 128-bit mask (two words), and the index of the row's first stick (sticks are sorted by
   // j * n3 + k, so stick (j, k) = rowstart + number of set bits of the mask below k).
@@ -62,7 +63,8 @@ struct Plan {
   // launch geometry
   int stick_w[2] = {0, 0};         // sticks per workgroup, [prec]
   int stick_threads[2] = {0, 0};   // workgroup size of the stick kernels
-  int plane_threads[2] = {0, 0};   // workgroup size of the plane kernel
+  int plane_threads[2] = {0, 0};   // workgroup size of the plane kernel (0: full-grid route)
+  int plane_fit[2] = {0, 0};       // the plane kernel's size where it fits (amt_plan_set_route restores it)
 };
 
 void set_error(const std::string& msg);
@@ -87,6 +89,14 @@ int hpsi_local_noalloc(const Plan& p, int prec, const void* psi, void* hpsi, int
 int launch_density(const Plan& p, int prec, const void* psi, int nb, int k0, int nkk, const int* d_band_list,
                    const double* d_band_weight, int nlist, int chunk, void* sticks, double* part, double* rho,
                    double beta, double scale, hipStream_t s);
+// The full-grid route (whole grids, a chunk of rows at a time in `ws`) for planes that do not
+// fit one workgroup's LDS; full_grid_rows = grids the workspace holds.
+long long full_grid_rows(const Plan& p, int prec, size_t ws_bytes);
+int launch_hpsi_full_grid(const Plan& p, int prec, const void* psi, void* hpsi, int nb, int k0, int nkk,
+                          const void* veff, void* ws, size_t ws_bytes, hipStream_t s);
+int launch_density_full_grid(const Plan& p, int prec, const void* psi, int nb, int k0, const int* list,
+                             const double* weight, int nlist, void* ws, size_t ws_bytes, double* acc, double* rho,
+                             double beta, double scale, hipStream_t s);
 int launch_scatter(const Plan& p, int prec, const void* psi, void* grid, int nb, int k, hipStream_t s);
 int launch_gather(const Plan& p, int prec, const void* grid, void* out, int nb, int k, double scale,
                   hipStream_t s);

@@ -191,7 +191,7 @@ class DevicePlan:
                 raise AmtError("hpsi: out must be a contiguous block shaped like psi (zero padding)")
         if nb == 0:
             return h[0] if single else h
-        if self.plane_ok[prec] and coupling is None:
+        if coupling is None:
             # the whole operator in one library call (local part + projector GEMMs)
             self._attach(prec)
             nbytes = lib().amt_hpsi_workspace(self._handle, prec, nb, nkk)
@@ -199,12 +199,9 @@ class DevicePlan:
             check(lib().amt_hpsi(self._handle, prec, p3.data_ptr(), h.data_ptr(), nb, k0, nkk, v.data_ptr(),
                                  ws.data_ptr(), nbytes, ops.stream_handle()), "hpsi")
             return h[0] if single else h
-        if self.plane_ok[prec]:
-            st = self.sticks(prec, nkk * nb)
-            check(lib().amt_hpsi_local(self._handle, prec, p3.data_ptr(), h.data_ptr(), nb, k0, nkk, v.data_ptr(),
-                                       st.data_ptr(), ops.stream_handle()), "hpsi_local")
-        else:
-            self._hpsi_local_full_grid(p3, h, v, k0)
+        st = self.sticks(prec, nkk * nb)
+        check(lib().amt_hpsi_local(self._handle, prec, p3.data_ptr(), h.data_ptr(), nb, k0, nkk, v.data_ptr(),
+                                   st.data_ptr(), ops.stream_handle()), "hpsi_local")
         kb = self.kb_tables(prec, coupling)
         if kb is not None:
             mat, dij, start, length = kb
@@ -218,57 +215,14 @@ class DevicePlan:
             ops.zgemm(pd, m, 0, alpha=1.0, beta=1.0, out=h)  # += (P D) beta
         return h[0] if single else h
 
-    # ------------------------------------------- grids whose plane does not fit LDS
-    def _band_chunk(self, itemsize: int) -> int:
-        """Bands per chunk of full grids (about 1 GiB of them)."""
-        n = self.shape[0] * self.shape[1] * self.shape[2]
-        return max(1, (1 << 30) // (n * itemsize))
-
-    def _hpsi_local_full_grid(self, p3: torch.Tensor, h: torch.Tensor, v: torch.Tensor, k0: int) -> None:
-        """T + gather(FFT(V IFFT(scatter psi))) for a grid whose (n2 x n3) plane does not fit one
-        workgroup's LDS at this precision (FP64 planes above ~100^2): the library's scatter, full
-        3D transforms and gather on whole grids, a bounded chunk of bands at a time. Same
-        transform pair and 1/N as the pruned path (``operator.py:64-69``, ``fft.py:219-235``)."""
-        nkk, nb, _ = p3.shape
-        prec = ops.precision_of(p3)
-        n = self.shape[0] * self.shape[1] * self.shape[2]
-        vg = v.reshape(self.shape)
-        step = self._band_chunk(p3.element_size())
-        for kk in range(nkk):
-            k = k0 + kk
-            kin = self.kinetic[prec][k]
-            for b0 in range(0, nb, step):
-                blk = p3[kk, b0 : b0 + step]
-                g = self.scatter(blk, k)
-                if g.dim() == 3:
-                    g = g.unsqueeze(0)
-                ops.fft3d(g, +1, 1.0, inplace=True)
-                g.mul_(vg)
-                ops.fft3d(g, -1, 1.0, inplace=True)
-                h[kk, b0 : b0 + step] = self.gather(g, k, scale=1.0 / n) + kin * blk
-
-    def _density_full_grid(self, p3: torch.Tensor, rows: np.ndarray, weights: np.ndarray, rho: torch.Tensor,
-                           k0: int, beta: float, scale: float) -> None:
-        """rho = beta rho + scale sum_l w_l |u_l(r)|^2 through full-grid transforms (FP64 sums),
-        for grids whose plane does not fit one workgroup's LDS at this precision."""
-        nkk, nb, _ = p3.shape
-        acc = torch.zeros(self.shape, dtype=torch.float64, device=self.device)
-        step = self._band_chunk(p3.element_size())
-        rows = np.asarray(rows, dtype=np.int64)
-        weights = np.asarray(weights, dtype=np.float64)
-        for kk in range(nkk):
-            sel = np.flatnonzero(rows // nb == kk)
-            for c0 in range(0, len(sel), step):
-                part = sel[c0 : c0 + step]
-                bands = torch.as_tensor(rows[part] % nb, device=self.device)
-                g = self.scatter(p3[kk].index_select(0, bands), k0 + kk)
-                if g.dim() == 3:
-                    g = g.unsqueeze(0)
-                ops.fft3d(g, +1, 1.0, inplace=True)
-                w = torch.as_tensor(weights[part], device=self.device)
-                acc += torch.einsum("b,bxyz->xyz", w, (g.real.double() ** 2 + g.imag.double() ** 2))
-        flat = rho.view(-1)
-        flat.mul_(beta).add_(acc.view(-1), alpha=scale)
+    def set_route(self, prec: int, full_grid: bool) -> None:
+        """Send this precision's local H.psi and density through the library's full-grid route
+        (``full_grid=True``) even where the plane kernels fit, or back (amt_plan_set_route)."""
+        check(lib().amt_plan_set_route(self._handle, prec, 1 if full_grid else 0), "plan_set_route")
+        info = (ctypes.c_int * 12)()
+        check(lib().amt_plan_info(self._handle, info, 12), "plan_info")
+        self.info = list(info)
+        self.plane_ok = {FP32: self.info[10] > 0, FP64: self.info[11] > 0}
 
     # ---------------------------------------------------------------- density
     def accumulate_density(self, psi: torch.Tensor, rows: np.ndarray, weights: np.ndarray, rho: torch.Tensor,
@@ -291,9 +245,6 @@ class DevicePlan:
             raise AmtError("density: block does not match the plan")
         if rho.numel() != self.shape[0] * self.shape[1] * self.shape[2] or not rho.is_cuda:
             raise AmtError("density: rho must be one device grid")
-        if not self.plane_ok[prec]:
-            self._density_full_grid(p3, rows, weights, rho, k0, beta, scale)
-            return
         lst = torch.as_tensor(rows, device=self.device)
         wgt = torch.as_tensor(np.asarray(weights, dtype=np.float64), device=self.device)
         st = self.sticks(prec, nkk * nb)

@@ -52,8 +52,16 @@ int amt_plan_destroy(amt_plan_t* plan);
  * plane_threads{fp32,fp64} (0 = plane does not fit LDS at that precision). */
 int amt_plan_info(const amt_plan_t* plan, int* info, int len);
 int amt_plan_nsticks(const amt_plan_t* plan, int* out);
-/* Bytes of the stick workspace for `nbands` band rows (all k of one launch). */
+/* Bytes of the local H.psi / density workspace for `nbands` band rows (all k of one launch):
+ * their sticks, or, where the (n2 x n3) plane does not fit one workgroup's LDS at this
+ * precision (info[10 + precision] = 0, e.g. FP64 planes above ~100^2), whole grids for
+ * min(nbands, ~1 GiB of grids) rows -- the full-grid route (the plan's scatter, 3-D
+ * transforms and gather, a chunk of rows at a time). */
 size_t amt_plan_stick_bytes(const amt_plan_t* plan, int precision, int nbands);
+/* Tuning hook, no counterpart in the reference: full_grid = 1 sends this precision's local
+ * H.psi and density through the full-grid route even where the plane fits (for comparison);
+ * 0 restores the plane kernels where they fit. Query amt_plan_stick_bytes again afterwards. */
+int amt_plan_set_route(amt_plan_t* plan, int precision, int full_grid);
 
 /*
  * Local part of H.psi for nb bands at each of the k-points k0..k0+nk-1:
@@ -62,7 +70,9 @@ size_t amt_plan_stick_bytes(const amt_plan_t* plan, int precision, int nbands);
  * i.e. apply_kinetic (hamiltonian/kinetic.py:22-24), FFTGrid.wave_to_real and
  * FFTGrid.real_to_wave (basis/fft.py:237-243). V_eff is one spin channel, N reals of
  * `precision`. psi/hpsi: [nk][nb][npw_pad]; padded columns of hpsi are not written.
- * The non-local term is added afterwards with amt_zgemm / amt_kb_dij.
+ * `sticks` holds amt_plan_stick_bytes(plan, precision, nb * nk) bytes. Pruned stick/plane
+ * transforms where the plane fits LDS, the full-grid route elsewhere.
+ * The non-local term is added afterwards with amt_zgemm / amt_kb_dij (or use amt_hpsi).
  */
 int amt_hpsi_local(const amt_plan_t* plan, int precision, const void* psi, void* hpsi, int nb, int k0, int nk,
                    const void* veff, void* sticks, void* stream);

@@ -195,7 +195,8 @@ def test_full_grid_route_matches_the_plane_kernels(si2_system, precision):
     rho_b = torch.zeros_like(rho_a)
     ref = plan.hpsi(psi, v)
     plan.accumulate_density(psi, rows, wts, rho_a)
-    plan.plane_ok[prec] = False
+    plan.set_route(prec, True)
+    assert not plan.plane_ok[prec]
     got = plan.hpsi(psi, v)
     plan.accumulate_density(psi, rows, wts, rho_b)
     tol = 1e-12 if precision == "fp64" else 2e-6
