@@ -29,6 +29,7 @@ SIGNATURES = {
     "amt_plan_nsticks": (_c_int, [_vp, _ip]),
     "amt_plan_stick_bytes": (_c_sz, [_vp, _c_int, _c_int]),
     "amt_plan_set_route": (_c_int, [_vp, _c_int, _c_int]),
+    "amt_plan_set_launch": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "amt_hpsi_local": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "amt_plan_set_projectors": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp]),
     "amt_hpsi_workspace": (_c_sz, [_vp, _c_int, _c_int, _c_int]),

@@ -1,4 +1,5 @@
-"""Measured choice of the split along k for the complex GEMMs.
+"""Measured choices of launch parameters: the split along k for the complex GEMMs and the
+launch shapes of the compile-time-length H.psi kernels.
 
 ``amt_zgemm`` splits a long-k product over several workgroups per output tile when the
 output alone has too few tiles to fill the GPU, with a deterministic second-pass
@@ -94,4 +95,76 @@ def tune_zgemm(run, prec: int, op: int, m: int, n: int, k: int, batch: int) -> i
     return best
 
 
-__all__ = ["MIN_GFLOP", "NEIGHBOURS", "bucket", "enabled", "heuristic_split", "tune_zgemm", "tuned_splits"]
+# ----------------------------------------------------------------------- FFT launch shapes
+#: (grid length, precision) -> (plane workgroup, sticks per workgroup) chosen so far
+_FFT: dict = {}
+
+
+def fft_enabled() -> bool:
+    """Launch shapes of the compile-time-length H.psi kernels are tuned unless AMT_AUTOTUNE_FFT=0."""
+    return os.environ.get("AMT_AUTOTUNE_FFT", "1") != "0"
+
+
+def tuned_fft_launches() -> dict:
+    return dict(_FFT)
+
+
+def _time_stage(plan, prec, stage, psi, out, v, st, k0, reps=REPEATS) -> float:
+    from . import ops
+
+    nkk, nb, _ = psi.shape
+
+    def run():
+        check(lib().amt_hpsi_stage(plan.handle, prec, stage, psi.data_ptr(), out.data_ptr(), nb, k0, nkk,
+                                   v.data_ptr(), st.data_ptr(), ops.stream_handle()), "hpsi_stage")
+
+    run()
+    stream = torch.cuda.current_stream()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        run()
+    stop.record(stream)
+    stop.synchronize()
+    return start.elapsed_time(stop) / reps
+
+
+def tune_fft_launch(plan, prec: int, psi: torch.Tensor, v: torch.Tensor, k0: int = 0) -> tuple:
+    """Pick the plane kernel's workgroup (256 or 512 threads) and the stick kernels' width (the
+    default or half) for this grid length and precision by timing each candidate on the
+    caller's block (read only; sticks and output go to scratch), with HIP events on the current
+    stream, and set them on the plan (``amt_plan_set_launch``). The choice is remembered per
+    (length, precision) and applied to later plans of that length without timing again. The
+    result of H.psi does not depend on it (every stick and plane is transformed by the same
+    arithmetic whatever the grouping)."""
+    key = (plan.shape[0], prec)
+    info = plan.info
+    if key not in _FFT:
+        info[12 + prec] = info[14 + prec] = -1  # the timing below leaves the plan on its last candidate
+        nkk, nb, _ = psi.shape
+        wdef = 32 if prec == 1 else 64
+        st = torch.empty(lib().amt_plan_stick_bytes(plan.handle, prec, nkk * nb), dtype=torch.uint8,
+                         device=psi.device)
+        out = torch.zeros_like(psi)
+        times = {}
+        for w in (wdef, wdef // 2):
+            check(lib().amt_plan_set_launch(plan.handle, prec, 0, w), "plan_set_launch")
+            times[("w", w)] = _time_stage(plan, prec, 1, psi, out, v, st, k0) + \
+                _time_stage(plan, prec, 3, psi, out, v, st, k0)
+        for nth in (256, 512):
+            check(lib().amt_plan_set_launch(plan.handle, prec, nth, 0), "plan_set_launch")
+            _time_stage(plan, prec, 1, psi, out, v, st, k0, reps=1)  # fresh sticks for the in-place plane pass
+            times[("p", nth)] = _time_stage(plan, prec, 2, psi, out, v, st, k0)
+        best_w = min((w for (kind, w) in times if kind == "w"), key=lambda w: times[("w", w)])
+        best_p = min((n for (kind, n) in times if kind == "p"), key=lambda n: times[("p", n)])
+        _FFT[key] = (best_p, best_w)
+        del st, out
+    nth, w = _FFT[key]
+    if (info[12 + prec], info[14 + prec]) != (nth, w):
+        check(lib().amt_plan_set_launch(plan.handle, prec, nth, w), "plan_set_launch")
+        info[12 + prec], info[14 + prec] = nth, w
+    return nth, w
+
+
+__all__ = ["MIN_GFLOP", "NEIGHBOURS", "bucket", "enabled", "fft_enabled", "heuristic_split", "tune_fft_launch",
+           "tune_zgemm", "tuned_fft_launches", "tuned_splits"]

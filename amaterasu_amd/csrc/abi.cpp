@@ -299,9 +299,35 @@ int amt_plan_info(const amt_plan_t* h, int* info, int len) {
     return 1;
   }
   const Plan& p = h->p;
-  const int v[12] = {p.n1, p.n2, p.n3, p.nk, p.npw_pad, p.smax, p.stick_w[0], p.stick_w[1], p.stick_threads[0],
-                     p.stick_threads[1], p.plane_threads[0], p.plane_threads[1]};
-  for (int i = 0; i < len && i < 12; ++i) info[i] = v[i];
+  const int v[16] = {p.n1,           p.n2,           p.n3,
+                     p.nk,           p.npw_pad,      p.smax,
+                     p.stick_w[0],   p.stick_w[1],   p.stick_threads[0],
+                     p.stick_threads[1], p.plane_threads[0], p.plane_threads[1],
+                     p.fast ? p.plane_nth[0] : 0, p.fast ? p.plane_nth[1] : 0,
+                     p.fast ? p.fast_stick_w[0] : 0, p.fast ? p.fast_stick_w[1] : 0};
+  for (int i = 0; i < len && i < 16; ++i) info[i] = v[i];
+  return 0;
+}
+
+int amt_plan_set_launch(amt_plan_t* h, int precision, int plane_threads, int stick_width) {
+  if (!h || (precision != kFP32 && precision != kFP64)) {
+    set_error("plan_set_launch: null plan or bad precision");
+    return 1;
+  }
+  Plan& p = h->p;
+  if (!p.fast) {
+    set_error("plan_set_launch: only the compile-time-length kernels of cubic grids have these launch shapes");
+    return 1;
+  }
+  const int wdef = precision == kFP64 ? 32 : 64;
+  if ((plane_threads != 0 && plane_threads != 256 && plane_threads != 512) ||
+      (stick_width != 0 && stick_width != wdef && stick_width != wdef / 2)) {
+    set_error("plan_set_launch: plane_threads must be 256 or 512, stick_width " + std::to_string(wdef) + " or " +
+              std::to_string(wdef / 2) + " (0 keeps the current value)");
+    return 1;
+  }
+  if (plane_threads) p.plane_nth[precision] = plane_threads;
+  if (stick_width) p.fast_stick_w[precision] = stick_width;
   return 0;
 }
 

@@ -45,12 +45,11 @@ __device__ __forceinline__ bool stick_block(int nch, int nkb, int& chunk, int& k
 }
 inline unsigned stick_grid(int nch, int nkb) { return 8u * (unsigned)nch * (unsigned)((nkb + 7) / 8); }
 
-template <int N, typename C>
+template <int N, typename C, int W>
 __global__ __launch_bounds__(256) void k_stick_inverse_n(const C* __restrict__ psi, C* __restrict__ sticks,
                                                          const int* __restrict__ stick_map, int smax, int npw_pad,
                                                          int nb, int k0, const C* __restrict__ tw, int nch,
                                                          int nkb) {
-  constexpr int W = stick_width<C>();
   constexpr int T2 = ceil_div(W * Split<N>::N1, 256);
   __shared__ C tile[N * W];
   const int tid = threadIdx.x;
@@ -78,12 +77,11 @@ __global__ __launch_bounds__(256) void k_stick_inverse_n(const C* __restrict__ p
   }
 }
 
-template <int N, typename C, typename R>
+template <int N, typename C, typename R, int W>
 __global__ __launch_bounds__(256) void k_stick_forward_n(const C* __restrict__ sticks, const C* __restrict__ psi,
                                                          C* __restrict__ out, const int* __restrict__ stick_map,
                                                          const R* __restrict__ kin, int smax, int npw_pad, int nb,
                                                          int k0, const C* __restrict__ tw, R scale, int nch, int nkb) {
-  constexpr int W = stick_width<C>();
   constexpr int T2 = ceil_div(W * Split<N>::N1, 256);
   __shared__ C tile[N * W];
   const int tid = threadIdx.x;
@@ -148,15 +146,14 @@ __device__ __forceinline__ void twiddle_row(C (&u)[NA], int j2, const C* __restr
 //    first phase consumes exactly the N2 outputs one inverse task holds;
 //  * the forward phase along j writes only rows that hold sticks, and the last phase along k
 //    writes its outputs straight back to the sticks.
-template <int N, typename C, typename R>
-__global__ __launch_bounds__(256) void k_plane_vloc_n(C* __restrict__ sticks, const int* __restrict__ rows,
+template <int N, typename C, typename R, int NTH>
+__global__ __launch_bounds__(NTH) void k_plane_vloc_n(C* __restrict__ sticks, const int* __restrict__ rows,
                                                       const int* __restrict__ nrows,
                                                       const unsigned long long* __restrict__ rowmask,
                                                       const int* __restrict__ rowstart,
                                                       const unsigned long long* __restrict__ livemask,
                                                       const R* __restrict__ veff, int smax, int nb, int k0,
                                                       const C* __restrict__ tw) {
-  constexpr int NTH = 256;
   constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
   constexpr int TR = ceil_div(N * N1, NTH);  // rounds of the (row or column, k1) tasks
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
@@ -466,36 +463,55 @@ __global__ __launch_bounds__(512) void k_density_plane_n(const C* __restrict__ s
 }
 
 // ----------------------------------------------------------------------- launchers
+template <int N, typename R, int W>
+static int stick_pass(const Plan& p, int inverse, const void* psi_, void* hpsi_, void* sticks_, int nb, int k0,
+                      int nkb, hipStream_t s) {
+  using C = typename Cplx<R>::T;
+  constexpr int prec = sizeof(R) == 8 ? kFP64 : kFP32;
+  const C* tw = prec == kFP64 ? (const C*)p.d_tw64[0] : (const C*)p.d_tw32[0];
+  const R* kin = prec == kFP64 ? (const R*)p.d_kin64 : (const R*)p.d_kin32;
+  const int nch = (p.smax + W - 1) / W;
+  const dim3 grid(stick_grid(nch, nkb));
+  if (inverse)
+    hipLaunchKernelGGL((k_stick_inverse_n<N, C, W>), grid, dim3(256), 0, s, static_cast<const C*>(psi_),
+                       static_cast<C*>(sticks_), p.d_stick_map, p.smax, p.npw_pad, nb, k0, tw, nch, nkb);
+  else
+    hipLaunchKernelGGL((k_stick_forward_n<N, C, R, W>), grid, dim3(256), 0, s, static_cast<const C*>(sticks_),
+                       static_cast<const C*>(psi_), static_cast<C*>(hpsi_), p.d_stick_map, kin, p.smax, p.npw_pad,
+                       nb, k0, tw, (R)(1.0 / ((double)N * N * N)), nch, nkb);
+  return hipGetLastError() != hipSuccess ? 1 : 0;
+}
+
 template <int N, typename R>
 static int hpsi_fast_t(const Plan& p, const void* psi_, void* hpsi_, int nb, int k0, int nkk, const void* veff_,
                        void* sticks_, hipStream_t s, int stage) {
   using C = typename Cplx<R>::T;
   constexpr int prec = sizeof(R) == 8 ? kFP64 : kFP32;
   constexpr int W = stick_width<C>();
-  const C* psi = static_cast<const C*>(psi_);
-  C* hpsi = static_cast<C*>(hpsi_);
   C* sticks = static_cast<C*>(sticks_);
   const R* veff = static_cast<const R*>(veff_);
   const C* tw = prec == kFP64 ? (const C*)p.d_tw64[0] : (const C*)p.d_tw32[0];
-  const R* kin = prec == kFP64 ? (const R*)p.d_kin64 : (const R*)p.d_kin32;
   const int nkb = nb * nkk;
-  const int nch = (p.smax + W - 1) / W;
-  const dim3 sgrid(stick_grid(nch, nkb));
+  const bool half = p.fast_stick_w[prec] == W / 2;  // the launch shape the plan holds (autotuner)
   const size_t plds = (size_t)N * N * sizeof(C);
   if (stage == 0 || stage == 1) {
-    hipLaunchKernelGGL((k_stick_inverse_n<N, C>), sgrid, dim3(256), 0, s, psi, sticks, p.d_stick_map, p.smax,
-                       p.npw_pad, nb, k0, tw, nch, nkb);
-    if (hipGetLastError() != hipSuccess) return 1;
+    if (half ? stick_pass<N, R, W / 2>(p, 1, psi_, hpsi_, sticks_, nb, k0, nkb, s)
+             : stick_pass<N, R, W>(p, 1, psi_, hpsi_, sticks_, nb, k0, nkb, s))
+      return 1;
   }
   if (stage == 0 || stage == 2) {
-    hipLaunchKernelGGL((k_plane_vloc_n<N, C, R>), dim3(N, nkb), dim3(256), plds, s, sticks, p.d_rows, p.d_nrows,
-                       p.d_rowmask, p.d_rowstart, p.d_livemask, veff, p.smax, nb, k0, tw);
+    if (p.plane_nth[prec] == 512)
+      hipLaunchKernelGGL((k_plane_vloc_n<N, C, R, 512>), dim3(N, nkb), dim3(512), plds, s, sticks, p.d_rows,
+                         p.d_nrows, p.d_rowmask, p.d_rowstart, p.d_livemask, veff, p.smax, nb, k0, tw);
+    else
+      hipLaunchKernelGGL((k_plane_vloc_n<N, C, R, 256>), dim3(N, nkb), dim3(256), plds, s, sticks, p.d_rows,
+                         p.d_nrows, p.d_rowmask, p.d_rowstart, p.d_livemask, veff, p.smax, nb, k0, tw);
     if (hipGetLastError() != hipSuccess) return 1;
   }
   if (stage == 0 || stage == 3) {
-    hipLaunchKernelGGL((k_stick_forward_n<N, C, R>), sgrid, dim3(256), 0, s, sticks, psi, hpsi, p.d_stick_map, kin,
-                       p.smax, p.npw_pad, nb, k0, tw, (R)(1.0 / ((double)N * N * N)), nch, nkb);
-    if (hipGetLastError() != hipSuccess) return 1;
+    if (half ? stick_pass<N, R, W / 2>(p, 0, psi_, hpsi_, sticks_, nb, k0, nkb, s)
+             : stick_pass<N, R, W>(p, 0, psi_, hpsi_, sticks_, nb, k0, nkb, s))
+      return 1;
   }
   return 0;
 }
@@ -509,11 +525,10 @@ static int density_fast_t(const Plan& p, const void* psi_, int nb, int k0, int n
   const C* tw = prec == kFP64 ? (const C*)p.d_tw64[0] : (const C*)p.d_tw32[0];
   const int nkb = nb * nkk;
   const int nchunk = (nlist + chunk - 1) / chunk;
-  const int nch = (p.smax + W - 1) / W;
-  hipLaunchKernelGGL((k_stick_inverse_n<N, C>), dim3(stick_grid(nch, nkb)), dim3(256), 0, s,
-                     static_cast<const C*>(psi_), static_cast<C*>(sticks_), p.d_stick_map, p.smax, p.npw_pad, nb,
-                     k0, tw, nch, nkb);
-  if (hipGetLastError() != hipSuccess) return 1;
+  const bool half = p.fast_stick_w[prec] == W / 2;  // the launch shape the plan holds (autotuner)
+  if (half ? stick_pass<N, R, W / 2>(p, 1, psi_, nullptr, sticks_, nb, k0, nkb, s)
+           : stick_pass<N, R, W>(p, 1, psi_, nullptr, sticks_, nb, k0, nkb, s))
+    return 1;
   hipLaunchKernelGGL((k_density_plane_n<N, C>), dim3(N, nchunk), dim3(512), (size_t)N * N * sizeof(C), s,
                      static_cast<const C*>(sticks_), p.d_rows, p.d_nrows, p.d_rowmask, p.d_rowstart, p.d_livemask,
                      list, weight, nlist, chunk, p.smax, nb, k0, tw, part);

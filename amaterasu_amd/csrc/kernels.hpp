@@ -65,6 +65,9 @@ struct Plan {
   int stick_threads[2] = {0, 0};   // workgroup size of the stick kernels
   int plane_threads[2] = {0, 0};   // workgroup size of the plane kernel (0: full-grid route)
   int plane_fit[2] = {0, 0};       // the plane kernel's size where it fits (amt_plan_set_route restores it)
+  // launch shapes of the compile-time-length kernels (amt_plan_set_launch; an autotuner picks them)
+  int plane_nth[2] = {256, 256};   // plane kernel workgroup: 256 or 512 threads
+  int fast_stick_w[2] = {64, 32};  // sticks per workgroup of the stick kernels: the default or half
 };
 
 void set_error(const std::string& msg);

@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import ops
+from . import autotune, ops
 from ._lib import FP32, FP64, AmtError, check, lib
 
 
@@ -64,8 +64,8 @@ class DevicePlan:
                                     miller.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                     kin.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "plan_create")
         self._handle = handle
-        info = (ctypes.c_int * 12)()
-        check(lib().amt_plan_info(handle, info, 12), "plan_info")
+        info = (ctypes.c_int * 16)()
+        check(lib().amt_plan_info(handle, info, 16), "plan_info")
         self.info = list(info)
         self.smax = self.info[5]
         self.plane_ok = {FP32: self.info[10] > 0, FP64: self.info[11] > 0}
@@ -191,6 +191,8 @@ class DevicePlan:
                 raise AmtError("hpsi: out must be a contiguous block shaped like psi (zero padding)")
         if nb == 0:
             return h[0] if single else h
+        if autotune.fft_enabled() and self.info[12 + prec] and self.plane_ok[prec]:
+            autotune.tune_fft_launch(self, prec, p3, v, k0)
         if coupling is None:
             # the whole operator in one library call (local part + projector GEMMs)
             self._attach(prec)
@@ -219,8 +221,8 @@ class DevicePlan:
         """Send this precision's local H.psi and density through the library's full-grid route
         (``full_grid=True``) even where the plane kernels fit, or back (amt_plan_set_route)."""
         check(lib().amt_plan_set_route(self._handle, prec, 1 if full_grid else 0), "plan_set_route")
-        info = (ctypes.c_int * 12)()
-        check(lib().amt_plan_info(self._handle, info, 12), "plan_info")
+        info = (ctypes.c_int * 16)()
+        check(lib().amt_plan_info(self._handle, info, 16), "plan_info")
         self.info = list(info)
         self.plane_ok = {FP32: self.info[10] > 0, FP64: self.info[11] > 0}
 

@@ -48,9 +48,17 @@ int amt_device_info(char* buf, int len);
 int amt_plan_create(amt_plan_t** out, int n1, int n2, int n3, int nk, const int* npw, int npw_pad,
                     const int* miller, const double* kinetic);
 int amt_plan_destroy(amt_plan_t* plan);
-/* info[0..11] = n1, n2, n3, nk, npw_pad, smax, stick_w{fp32,fp64}, stick_threads{fp32,fp64},
- * plane_threads{fp32,fp64} (0 = plane does not fit LDS at that precision). */
+/* info[0..15] = n1, n2, n3, nk, npw_pad, smax, stick_w{fp32,fp64}, stick_threads{fp32,fp64},
+ * plane_threads{fp32,fp64} (0 = plane does not fit LDS at that precision: full-grid route),
+ * and for cubic grids with compile-time-length kernels (else 0) the launch shapes of
+ * amt_plan_set_launch: plane workgroup{fp32,fp64}, sticks per workgroup{fp32,fp64}. */
 int amt_plan_info(const amt_plan_t* plan, int* info, int len);
+/* Tuning hook, no counterpart in the reference: the launch shape of the compile-time-length
+ * H.psi kernels at `precision` -- plane kernel workgroup of 256 or 512 threads, stick kernels
+ * over 64 or 32 sticks per workgroup (fp32; 32 or 16 in fp64); 0 keeps a value. Results do
+ * not depend on it; amaterasu_amd.autotune times the candidates per grid length and keeps
+ * the fastest. */
+int amt_plan_set_launch(amt_plan_t* plan, int precision, int plane_threads, int stick_width);
 int amt_plan_nsticks(const amt_plan_t* plan, int* out);
 /* Bytes of the local H.psi / density workspace for `nbands` band rows (all k of one launch):
  * their sticks, or, where the (n2 x n3) plane does not fit one workgroup's LDS at this

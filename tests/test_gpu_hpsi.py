@@ -267,3 +267,43 @@ def test_full_operator_entry_point_matches_the_call_sequence(si_pseudo, precisio
     with pytest.raises(Exception, match="workspace smaller"):
         check(lib().amt_hpsi(plan.handle, prec, psi.data_ptr(), bare.data_ptr(), 7, 0, 3, v.data_ptr(),
                              ws.data_ptr(), 16, stream), "hpsi")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_launch_shapes_give_identical_results(si_pseudo, precision):
+    """Every launch shape the autotuner may pick (plane kernel of 256 or 512 threads, stick
+    kernels over the default or half the sticks per workgroup) gives bit-identical H.psi on
+    the Si8 k-batch (42^3); the hook rejects other shapes."""
+    import torch
+
+    from amaterasu_amd import ops
+    from amaterasu_amd._lib import AmtError, check, lib
+    from amaterasu_amd.device import DevicePlan
+    from amaterasu_amd.system import build_system, load_dataset
+    from amaterasu_amd.workloads import seeded_coefficients, si8_kbatch, smooth_potential
+
+    spec = si8_kbatch(nk=3, nbands=9)
+    s = build_system(spec.cell, spec.positions, spec.symbols, {"Si": load_dataset()}, spec.ecut_rho, spec.kpoints,
+                     spec.ecuts)
+    plan = DevicePlan(s.grid.shape, s.wave_spheres, s.projectors)
+    prec = 1 if precision == "fp64" else 0
+    cdt = torch.complex128 if prec else torch.complex64
+    psi = torch.zeros((3, 9, plan.npw_pad), dtype=cdt, device="cuda")
+    for k, sp in enumerate(s.wave_spheres):
+        psi[k, :, : len(sp.g2)] = torch.from_numpy(seeded_coefficients(sp.g2, 9, seed=60 + k)).to("cuda", cdt)
+    v = torch.from_numpy(smooth_potential(s.grid.shape, seed=6)).to("cuda", ops.real_dtype(prec))
+    st = plan.sticks(prec, 27)
+    wdef = 32 if prec else 64
+    outs = []
+    for nth in (256, 512):
+        for w in (wdef, wdef // 2):
+            check(lib().amt_plan_set_launch(plan.handle, prec, nth, w), "plan_set_launch")
+            out = torch.zeros_like(psi)
+            check(lib().amt_hpsi_local(plan.handle, prec, psi.data_ptr(), out.data_ptr(), 9, 0, 3, v.data_ptr(),
+                                       st.data_ptr(), ops.stream_handle()), "hpsi_local")
+            outs.append(out)
+    torch.cuda.synchronize()
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
+    with pytest.raises(AmtError, match="plane_threads must be"):
+        check(lib().amt_plan_set_launch(plan.handle, prec, 384, 0), "plan_set_launch")
