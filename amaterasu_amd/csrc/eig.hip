@@ -41,6 +41,10 @@ constexpr double kClusterGap = 2e-4;  // cluster threshold, relative to the Gers
 constexpr int kInverseRounds = 3;
 }  // namespace eig
 
+// Steps of a serial tridiagonal chain whose independent LDS loads are issued together (the
+// chain itself stays in registers, so LDS latency is paid once per chunk).
+constexpr int kTriChunk = 8;
+
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) { return cmul(a, b); }
 __device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
 
@@ -592,15 +596,26 @@ __global__ __launch_bounds__(eig::NT) void k_tri_bisect(const double* __restrict
     if (fabs(q0) < pivmin) q0 = -pivmin;
     if (fabs(q1) < pivmin) q1 = -pivmin;
     int c0 = q0 < 0.0, c1 = q1 < 0.0;
-    for (int i = 1; i < n; ++i) {
-      const double di = dl[i], ei = e2[i - 1];
+    auto sturm = [&](double di, double ei) {
       q0 = (di - x0) - ei * recip(q0);
       q1 = (di - x1) - ei * recip(q1);
       if (fabs(q0) < pivmin) q0 = -pivmin;
       if (fabs(q1) < pivmin) q1 = -pivmin;
       c0 += q0 < 0.0;
       c1 += q1 < 0.0;
+    };
+    int i = 1;
+    for (; i + kTriChunk <= n; i += kTriChunk) {  // the (broadcast) LDS loads of a chunk issued at once
+      double dd[kTriChunk], ee[kTriChunk];
+#pragma unroll
+      for (int u = 0; u < kTriChunk; ++u) {
+        dd[u] = dl[i + u];
+        ee[u] = e2[i - 1 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < kTriChunk; ++u) sturm(dd[u], ee[u]);
     }
+    for (; i < n; ++i) sturm(dl[i], e2[i - 1]);
     // the first point (0-based j, x_j = lo + width (j+1)/(P+1)) whose count exceeds k
     int f = c0 >= k + 1 ? 2 * tid : (c1 >= k + 1 ? 2 * tid + 1 : P);
     for (int off = 32; off > 0; off >>= 1) f = min(f, __shfl_xor(f, off, 64));
@@ -631,34 +646,49 @@ struct TriLU {
   double *rinv, *u1, *u2, *l, *sw;
 };
 
+// Both routines below are serial chains run by one thread over LDS-resident arrays. The chain
+// itself lives in registers (a, b in the factorisation; the running element in the solves);
+// every load the chain does not depend on is issued kTriChunk steps ahead.
 __device__ void tri_factor(const double* __restrict__ dl, const double* __restrict__ el, int n, double lam,
                            double tiny, const TriLU& f) {
   double a = dl[0] - lam, b = n > 1 ? el[0] : 0.0;
-  for (int i = 0; i + 1 < n; ++i) {
-    const double s = el[i], dd = dl[i + 1] - lam, g = i + 2 < n ? el[i + 1] : 0.0;
+  int i = 0;
+  auto step = [&](int ii, double s, double dd, double g) {
     double p;
     if (fabs(s) > fabs(a)) {
       const double mlt = a / s;
       p = s;
-      f.u1[i] = dd;
-      f.u2[i] = g;
-      f.l[i] = mlt;
-      f.sw[i] = 1.0;
+      f.u1[ii] = dd;
+      f.u2[ii] = g;
+      f.l[ii] = mlt;
+      f.sw[ii] = 1.0;
       a = b - mlt * dd;
       b = -mlt * g;
     } else {
       const double mlt = a != 0.0 ? s / a : 0.0;
       p = a;
-      f.u1[i] = b;
-      f.u2[i] = 0.0;
-      f.l[i] = mlt;
-      f.sw[i] = 0.0;
+      f.u1[ii] = b;
+      f.u2[ii] = 0.0;
+      f.l[ii] = mlt;
+      f.sw[ii] = 0.0;
       a = dd - mlt * b;
       b = g;
     }
     if (fabs(p) < tiny) p = p < 0.0 ? -tiny : tiny;
-    f.rinv[i] = 1.0 / p;
+    f.rinv[ii] = 1.0 / p;
+  };
+  for (; i + kTriChunk + 1 < n; i += kTriChunk) {  // steps i..i+7 read el[i..i+8], dl[i+1..i+8]
+    double s[kTriChunk], dd[kTriChunk], g[kTriChunk];
+#pragma unroll
+    for (int u = 0; u < kTriChunk; ++u) {
+      s[u] = el[i + u];
+      dd[u] = dl[i + 1 + u] - lam;
+      g[u] = el[i + 1 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < kTriChunk; ++u) step(i + u, s[u], dd[u], g[u]);
   }
+  for (; i + 1 < n; ++i) step(i, el[i], dl[i + 1] - lam, i + 2 < n ? el[i + 1] : 0.0);
   if (fabs(a) < tiny) a = a < 0.0 ? -tiny : tiny;
   f.rinv[n - 1] = 1.0 / a;
   f.u1[n - 1] = 0.0;
@@ -669,19 +699,67 @@ __device__ void tri_factor(const double* __restrict__ dl, const double* __restri
 
 // x <- (T - lam I)^{-1} x with the factors.
 __device__ void tri_solve(const TriLU& f, int n, double* __restrict__ x) {
-  for (int i = 0; i + 1 < n; ++i) {
+  // forward elimination: step i reads the original x[i+1] and the running x[i] (cur) and
+  // writes the final x[i]; cur carries the new x[i+1]
+  double cur = x[0];
+  int i = 0;
+  for (; i + kTriChunk < n; i += kTriChunk) {
+    double o[kTriChunk], l[kTriChunk], sw[kTriChunk], out[kTriChunk];
+#pragma unroll
+    for (int u = 0; u < kTriChunk; ++u) {
+      o[u] = x[i + 1 + u];
+      l[u] = f.l[i + u];
+      sw[u] = f.sw[i + u];
+    }
+#pragma unroll
+    for (int u = 0; u < kTriChunk; ++u) {
+      if (sw[u] != 0.0) {
+        out[u] = o[u];
+        cur = cur - l[u] * o[u];
+      } else {
+        out[u] = cur;
+        cur = o[u] - l[u] * cur;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kTriChunk; ++u) x[i + u] = out[u];
+  }
+  for (; i + 1 < n; ++i) {
+    const double o = x[i + 1];
     if (f.sw[i] != 0.0) {
-      const double t = x[i];
-      x[i] = x[i + 1];
-      x[i + 1] = t - f.l[i] * x[i + 1];
+      x[i] = o;
+      cur = cur - f.l[i] * o;
     } else {
-      x[i + 1] -= f.l[i] * x[i];
+      x[i] = cur;
+      cur = o - f.l[i] * cur;
     }
   }
-  double x1 = 0.0, x2 = 0.0;  // x[i+1], x[i+2]
-  for (int i = n - 1; i >= 0; --i) {
-    const double v = (x[i] - f.u1[i] * x1 - f.u2[i] * x2) * f.rinv[i];
-    x[i] = v;
+  x[n - 1] = cur;
+  // back substitution from the end, x1 = x[i+1], x2 = x[i+2] in registers
+  double x1 = 0.0, x2 = 0.0;
+  int j = n - 1;
+  for (; j - kTriChunk + 1 >= 0; j -= kTriChunk) {
+    double xv[kTriChunk], u1[kTriChunk], u2[kTriChunk], ri[kTriChunk];
+#pragma unroll
+    for (int u = 0; u < kTriChunk; ++u) {
+      xv[u] = x[j - u];
+      u1[u] = f.u1[j - u];
+      u2[u] = f.u2[j - u];
+      ri[u] = f.rinv[j - u];
+    }
+#pragma unroll
+    for (int u = 0; u < kTriChunk; ++u) {
+      const double v = (xv[u] - u1[u] * x1 - u2[u] * x2) * ri[u];
+      xv[u] = v;
+      x2 = x1;
+      x1 = v;
+    }
+#pragma unroll
+    for (int u = 0; u < kTriChunk; ++u) x[j - u] = xv[u];
+  }
+  for (; j >= 0; --j) {
+    const double v = (x[j] - f.u1[j] * x1 - f.u2[j] * x2) * f.rinv[j];
+    x[j] = v;
     x2 = x1;
     x1 = v;
   }
