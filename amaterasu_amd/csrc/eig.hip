@@ -25,8 +25,9 @@
 //       makes them eigenvectors, so near-degenerate and degenerate eigenvalues come out
 //       orthogonal with small residuals. Vectors of well-separated eigenvalues are accurate
 //       to about eps ||T|| / gap, below 1e-12 for gaps above the cluster threshold.
-// 3. Back-transformation Z = Q S for the lowest nev eigenvectors: one workgroup per
-//    eigenvector applies the reflectors in reverse order (no global synchronisation).
+// 3. Back-transformation Z = Q S for the lowest nev eigenvectors in compact-WY blocks of 32
+//    reflectors: the blocks' T factors from their Gram matrices, then one workgroup per two
+//    eigenvectors applies the blocks in reverse order (no global synchronisation).
 #include <cfloat>
 
 #include "kernels.hpp"
@@ -493,36 +494,189 @@ __global__ __launch_bounds__(eig::NT, 2) void k_hetrd_step_wide(double2* __restr
 }
 
 // Z^T[k][:] = Q S[:, k] for k < nev; S column-major (ldc = n), real.
-__global__ __launch_bounds__(eig::NT) void k_backtransform(const double2* __restrict__ V,
-                                                           const double2* __restrict__ taus,
-                                                           const double* __restrict__ S, int n,
-                                                           double2* __restrict__ Zt) {
-  double2* z = hetrd_dyn;  // n entries
-  __shared__ double red[2 * (eig::NT / 64) + 2];
-  const int k = blockIdx.x;
-  for (int i = threadIdx.x; i < n; i += eig::NT) z[i] = make_double2(S[(long long)k * n + i], 0.0);
+// Back-transformation in compact-WY blocks (LAPACK's zlarft/zlarfb, forward, columnwise):
+// the reflectors H_j0 .. H_{j0+kb-1} of one block multiply to I - V T V^H with T upper
+// triangular, T[i][i] = tau_i and T[0:i][i] = -tau_i T[0:i][0:i] (V[:, 0:i]^H v_i). Reflector
+// j is stored from V[j n] with element t at row j + 1 + t (v_j[0] = 1). Applying a block to
+// an eigenvector is two sums over rows and a kBtBlock x kBtBlock triangular product, so the
+// serial chain of one eigenvector is n / kBtBlock blocks instead of n - 1 reflectors.
+constexpr int kBtBlock = 32;
+constexpr int kBtRows = 64;  // rows per partial Gram workgroup
+constexpr int kBtVecs = 2;   // eigenvectors per back-transformation workgroup
+
+__host__ __device__ inline int bt_blocks(int n) { return (n - 1 + kBtBlock - 1) / kBtBlock; }
+__host__ __device__ inline int bt_chunks(int n) { return (n - 1 + kBtRows - 1) / kBtRows; }
+
+// Partial Gram matrices of the blocks: Gp[b][ch][a][c] = sum over rows r in chunk ch of
+// block b (r = j0 + 1 + ch kBtRows + rr) of conj(v_a[r]) v_c[r], grid (chunks, blocks).
+__global__ __launch_bounds__(eig::NT) void k_bt_gram(const double2* __restrict__ V, int n,
+                                                     double2* __restrict__ Gp) {
+  constexpr int B = kBtBlock, RC = kBtRows;
+  __shared__ double2 tile[RC][B + 1];
+  const int ch = blockIdx.x, b = blockIdx.y, j0 = b * B;
+  const int kb = min(B, n - 1 - j0);
+  const int r0 = j0 + 1 + ch * RC;
+  if (r0 >= n) return;  // past this block's rows (uniform over the workgroup)
+  for (int idx = threadIdx.x; idx < RC * B; idx += eig::NT) {
+    const int a = idx / RC, rr = idx % RC;  // consecutive threads: consecutive rows of one reflector
+    const int r = r0 + rr, t = r - j0 - a - 1;
+    tile[rr][a] = (a < kb && t >= 0 && r < n) ? V[(long long)(j0 + a) * n + t] : make_double2(0.0, 0.0);
+  }
   __syncthreads();
-  for (int j = n - 2; j >= 0; --j) {
-    const int len = n - j - 1;
-    const double2* v = V + (long long)j * n;
-    double2 part = make_double2(0.0, 0.0);
-    for (int t = threadIdx.x; t < len; t += eig::NT) {
-      const double2 c = zmul(zconj(v[t]), z[j + 1 + t]);
-      part.x += c.x;
-      part.y += c.y;
+  const int a0 = 2 * (threadIdx.x >> 4), c0 = 2 * (threadIdx.x & 15);
+  double2 acc[2][2] = {};
+  for (int rr = 0; rr < RC; ++rr) {
+    const double2 x0 = zconj(tile[rr][a0]), x1 = zconj(tile[rr][a0 + 1]);
+    const double2 y0 = tile[rr][c0], y1 = tile[rr][c0 + 1];
+    acc[0][0] = cadd(acc[0][0], zmul(x0, y0));
+    acc[0][1] = cadd(acc[0][1], zmul(x0, y1));
+    acc[1][0] = cadd(acc[1][0], zmul(x1, y0));
+    acc[1][1] = cadd(acc[1][1], zmul(x1, y1));
+  }
+  double2* g = Gp + ((long long)b * bt_chunks(n) + ch) * B * B;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) g[(a0 + i) * B + c0 + j] = acc[i][j];
+}
+
+// T of each block from the summed Gram entries and tau (one workgroup per block). The
+// recursion runs on one wave with lane a holding row a of T in registers (both loops
+// unrolled), reading column i of the Gram matrix from LDS.
+__global__ __launch_bounds__(eig::NT) void k_bt_tfactor(const double2* __restrict__ Gp,
+                                                        const double2* __restrict__ taus, int n,
+                                                        double2* __restrict__ Tf) {
+  constexpr int B = kBtBlock;
+  __shared__ double2 G[B][B + 1];
+  const int b = blockIdx.x, j0 = b * B;
+  const int kb = min(B, n - 1 - j0);
+  const int nch = (n - 1 - j0 + kBtRows - 1) / kBtRows;
+  const double2* gb = Gp + (long long)b * bt_chunks(n) * B * B;
+  for (int idx = threadIdx.x; idx < B * B; idx += eig::NT) {
+    const int a = idx / B, c = idx % B;
+    double2 s = make_double2(0.0, 0.0);
+    if (a < c && c < kb)
+      for (int ch = 0; ch < nch; ++ch) s = cadd(s, gb[(long long)ch * B * B + idx]);
+    G[a][c] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < B) {
+    const int a = threadIdx.x;
+    double2 t[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int c = 0; c < i; ++c)
+        if (c >= a) s = cadd(s, zmul(t[c], G[c][i]));
+      const double2 ti = i < kb ? taus[j0 + i] : make_double2(0.0, 0.0);
+      const double2 q = zmul(ti, s);
+      t[i] = a < i ? make_double2(-q.x, -q.y) : (a == i ? ti : make_double2(0.0, 0.0));
     }
-    const double2 y = block_sum2(part, red);
-    const double2 ty = zmul(taus[j], y);
-    for (int t = threadIdx.x; t < len; t += eig::NT) {
-      const double2 u = zmul(v[t], ty);
-      z[j + 1 + t].x -= u.x;
-      z[j + 1 + t].y -= u.y;
+    double2* out = Tf + ((long long)b * B + a) * B;
+#pragma unroll
+    for (int i = 0; i < B; ++i) out[i] = t[i];
+  }
+}
+
+// Z = Q S for kBtVecs eigenvectors per workgroup (kept in LDS): for the blocks in reverse
+// order, w = V^H z, u = T w, z -= V u. Both row sums issue their global loads in batches
+// (a batch of reflector elements per thread for w, a whole row of the block for the
+// update), so a block costs a few memory round trips rather than one per row.
+constexpr int kBtThreads = 512;
+constexpr int kBtSlices = kBtThreads / kBtBlock;  // threads per reflector in the w sums
+constexpr int kBtBatch = 8;    // loads per thread in flight in the update's row sums
+constexpr int kBtBatchW = 16;  // and in the sums w = V^H z
+
+__global__ __launch_bounds__(kBtThreads) void k_bt_apply(const double2* __restrict__ V,
+                                                         const double2* __restrict__ Tf,
+                                                         const double* __restrict__ S, int n, int nev,
+                                                         double2* __restrict__ Zt) {
+  constexpr int B = kBtBlock, E = kBtVecs, NT = kBtThreads, SL = kBtSlices;
+  double2* z = hetrd_dyn;  // E x n
+  __shared__ double2 w[E][B], u[E][B];
+  const int tid = threadIdx.x, e0 = blockIdx.x * E;
+  for (int i = tid; i < E * n; i += NT) {
+    const int e = i / n, r = i % n;
+    z[i] = make_double2(e0 + e < nev ? S[(long long)(e0 + e) * n + r] : 0.0, 0.0);
+  }
+  __syncthreads();
+  for (int b = bt_blocks(n) - 1; b >= 0; --b) {
+    const int j0 = b * B;
+    const int kb = min(B, n - 1 - j0);
+    {
+      const int a = tid / SL, sl = tid % SL;  // SL consecutive lanes per reflector
+      double2 acc[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) acc[e] = make_double2(0.0, 0.0);
+      if (a < kb) {
+        const double2* v = V + (long long)(j0 + a) * n;
+        const double2* zz = z + j0 + a + 1;
+        const int len = n - j0 - a - 1;
+        for (int t0 = sl; t0 < len; t0 += SL * kBtBatchW) {
+          double2 vv[kBtBatchW];
+#pragma unroll
+          for (int q = 0; q < kBtBatchW; ++q) {
+            const int t = t0 + SL * q;
+            vv[q] = t < len ? v[t] : make_double2(0.0, 0.0);
+          }
+#pragma unroll
+          for (int q = 0; q < kBtBatchW; ++q) {
+            const int t = t0 + SL * q;
+            if (t < len) {
+              const double2 cv = zconj(vv[q]);
+#pragma unroll
+              for (int e = 0; e < E; ++e) acc[e] = cadd(acc[e], zmul(cv, zz[e * n + t]));
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        for (int o = 1; o < SL; o <<= 1) {
+          acc[e].x += __shfl_xor(acc[e].x, o, 64);
+          acc[e].y += __shfl_xor(acc[e].y, o, 64);
+        }
+      if (sl == 0)
+#pragma unroll
+        for (int e = 0; e < E; ++e) w[e][a] = acc[e];
+    }
+    __syncthreads();
+    if (tid < E * B) {
+      const int e = tid / B, a = tid % B;
+      double2 s = make_double2(0.0, 0.0);
+      if (a < kb) {
+        const double2* t = Tf + ((long long)b * B + a) * B;
+        for (int c = a; c < kb; ++c) s = cadd(s, zmul(t[c], w[e][c]));
+      }
+      u[e][a] = s;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int r = j0 + 1 + tid; r < n; r += NT) {
+      const int amax = min(kb, r - j0);
+      double2 d[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) d[e] = make_double2(0.0, 0.0);
+#pragma unroll 1
+      for (int h = 0; h < B; h += kBtBatch) {
+        double2 vv[kBtBatch];
+#pragma unroll
+        for (int q = 0; q < kBtBatch; ++q)
+          vv[q] = h + q < amax ? V[(long long)(j0 + h + q) * n + (r - j0 - h - q - 1)] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+#pragma unroll
+          for (int q = 0; q < kBtBatch; ++q) d[e] = cadd(d[e], zmul(vv[q], u[e][h + q]));
+      }
+#pragma unroll
+      for (int e = 0; e < E; ++e) z[e * n + r] = csub(z[e * n + r], d[e]);
     }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < n; i += eig::NT) Zt[(long long)k * n + i] = z[i];
+  for (int i = tid; i < E * n; i += NT) {
+    const int e = i / n;
+    if (e0 + e < nev) Zt[(long long)(e0 + e) * n + i % n] = z[i];
+  }
 }
-
 
 // ------------------------------------------------------------ tridiagonal eigenpairs
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1061,9 +1215,11 @@ static void enqueue_tridiag(hipStream_t s, int n, double2* A, double2* V, double
 }
 
 size_t heev_workspace_bytes(int n) {
-  // A copy, V, p (2), taus, S (n x n real), d, e, bisection values, 4 scalars, LU factors (5 n x n)
+  // A copy, V, p (2), taus, S (n x n real), d, e, bisection values, 4 scalars, LU factors (5 n x n),
+  // partial Gram matrices and T factors of the back-transformation blocks
+  const size_t bb = n >= 2 ? (size_t)bt_blocks(n) : 0, bc = n >= 2 ? (size_t)bt_chunks(n) : 0;
   return (size_t)n * n * 16 * 2 + (size_t)2 * n * 16 + (size_t)n * 16 + (size_t)n * n * 8 + (size_t)3 * n * 8 + 32 +
-         (size_t)5 * n * n * 8;
+         (size_t)5 * n * n * 8 + 16 + bb * (bc + 1) * kBtBlock * kBtBlock * 16;
 }
 
 int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void* ws, size_t ws_bytes,
@@ -1110,8 +1266,12 @@ int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void*
     set_error("heev: tridiagonal eigenpair launch failed");
     return 1;
   }
-  hipLaunchKernelGGL(k_backtransform, dim3(nev), dim3(eig::NT), (size_t)n * sizeof(double2), s, V, taus, S, n,
-                     static_cast<double2*>(zt));
+  double2* Gp = reinterpret_cast<double2*>((reinterpret_cast<uintptr_t>(lu + (size_t)5 * n * n) + 15) & ~uintptr_t(15));
+  double2* Tf = Gp + (size_t)bt_blocks(n) * bt_chunks(n) * kBtBlock * kBtBlock;
+  hipLaunchKernelGGL(k_bt_gram, dim3(bt_chunks(n), bt_blocks(n)), dim3(eig::NT), 0, s, V, n, Gp);
+  hipLaunchKernelGGL(k_bt_tfactor, dim3(bt_blocks(n)), dim3(eig::NT), 0, s, Gp, taus, n, Tf);
+  hipLaunchKernelGGL(k_bt_apply, dim3((nev + kBtVecs - 1) / kBtVecs), dim3(kBtThreads),
+                     (size_t)kBtVecs * n * sizeof(double2), s, V, Tf, S, n, nev, static_cast<double2*>(zt));
   if (hipGetLastError() != hipSuccess) {
     set_error("heev: back-transformation launch failed");
     return 1;

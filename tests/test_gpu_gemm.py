@@ -130,4 +130,7 @@ def test_zgemm_fp32_large_tiles_long_contraction():
     rot = torch.randn((nb, 300, m), dtype=torch.complex128, generator=g).cuda() / m**0.5
     got = ops.zgemm(rot.to(torch.complex64), a, 0).to(torch.complex128)
     want = ops.zgemm(rot.to(torch.complex64).to(torch.complex128), a64, 0)
-    assert float((got - want).abs().max() / want.abs().max()) < 1e-6
+    # single-level FP32 over K = 256 in Gauss's three-product form: the imaginary part's
+    # rounding is bounded by eps * sum |a_re + a_im| |b_re + b_im|, up to twice the bound of
+    # four real products (measured 1.1e-6 here; the four-product form passed 1e-6)
+    assert float((got - want).abs().max() / want.abs().max()) < 2e-6
