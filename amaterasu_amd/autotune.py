@@ -22,6 +22,7 @@ converge) costs more than it saves unless the process runs long enough to reuse 
 
 from __future__ import annotations
 
+import math
 import os
 
 import torch
@@ -39,13 +40,19 @@ def enabled() -> bool:
     return os.environ.get("AMT_AUTOTUNE", "0") == "1"
 
 
-def heuristic_split(m: int, n: int, k: int, batch: int) -> int:
-    """The library's built-in choice (``choose_split`` in gemm.hip): about 1024 workgroups of
-    64 x 64 tiles, at least 256 of k per split, at most 256 splits."""
+def heuristic_split(m: int, n: int, k: int, batch: int, prec: int = 0) -> int:
+    """The library's built-in choice (``choose_split`` in gemm.hip): the smallest split giving
+    about 1024 workgroups of 64 x 64 tiles and, in FP32 (prec 0), K chunks of at most 1024;
+    then as many splits as fit in the same number of 1024-workgroup rounds; at least 256 of k
+    per split, at most 256 splits."""
     tiles = ((m + 63) // 64) * ((n + 63) // 64) * batch
     if tiles >= 512:
         return 1
-    return int(max(1, min((1024 + tiles - 1) // tiles, (k + 255) // 256, 256)))
+    single = (k + 1023) // 1024 if prec == 0 else 1
+    want = max((1024 + tiles - 1) // tiles, single)
+    rounds = max(1, math.ceil(tiles * want / 1024 - 0.05))
+    want = max(rounds * 1024 // tiles, single)
+    return int(max(1, min(want, (k + 255) // 256, 256)))
 
 
 def bucket(prec: int, op: int, m: int, n: int, k: int, batch: int) -> tuple:
@@ -67,7 +74,7 @@ def tune_zgemm(run, prec: int, op: int, m: int, n: int, k: int, batch: int) -> i
         _TUNED[key] = 0
         return 0
     kmax = max(1, (k + 255) // 256)
-    h = heuristic_split(m, n, k, batch)
+    h = heuristic_split(m, n, k, batch, prec)
     if h == 1 and kmax == 1:
         _TUNED[key] = 0
         return 0
