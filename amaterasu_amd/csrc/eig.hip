@@ -184,14 +184,15 @@ __device__ __forceinline__ double2 block_sum2_once(double2 v, double* red) {
 // sum p^H v, the publication of w, and the norm of the new column. The reflector of column
 // j+1 (zlarfg) is kept unscaled in LDS; its scale enters the product with the owned row
 // once, after the row's dot product, and the stored V row.
-// kRowRegs = elements of one row per lane: 16, for n up to 1024 (larger n: k_hetrd_step_wide).
+// kRowRegs = elements of one row per lane, 64 kRowRegs >= the trailing length n - j - 1 (4, 8, 12
+// or 16; longer rows: k_hetrd_step_wide).
 
 template <int kRowRegs>
 __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A, int n, int j,
                                                         double2* __restrict__ V, double2* __restrict__ taus,
                                                         double* __restrict__ d, double* __restrict__ e,
                                                         const double2* __restrict__ p_in, double2* __restrict__ p_out) {
-  constexpr int QV = kRowRegs / 4;  // vector elements per thread (n <= NT * QV)
+  constexpr int QV = kRowRegs / 4;  // vector elements per thread (n - j - 1 <= NT * QV)
   double2* sv = hetrd_dyn;          // v_j
   double2* sw = hetrd_dyn + n;      // w_j
   double2* sx = hetrd_dyn + 2 * n;  // column j+1 below the diagonal, unscaled
@@ -1202,15 +1203,25 @@ static void enqueue_tridiag(hipStream_t s, int n, double2* A, double2* V, double
   const int G = hetrd_groups(n);
   hipLaunchKernelGGL(k_hetrd_start, dim3(G), dim3(eig::NT), 3 * (size_t)n * sizeof(double2), s, A, n, V, taus,
                      d, e, p0);
+  // Step j touches the trailing rows j+2 .. n-1 and columns j+1 .. n-1 only: its grid covers
+  // those rows (one per wave) and its row template the trailing length, so the late, short
+  // steps launch few workgroups with short loops.
+  const size_t lds3 = 3 * (size_t)n * sizeof(double2), lds2 = 2 * (size_t)n * sizeof(double2);
   for (int j = 0; j <= n - 2; ++j) {
     const double2* pin = (j & 1) ? p1 : p0;
     double2* pout = (j & 1) ? p0 : p1;
-    if (n <= eig::MAXN / 2)
-      hipLaunchKernelGGL(k_hetrd_step<eig::MAXN / 128>, dim3(G), dim3(eig::NT), 3 * (size_t)n * sizeof(double2), s,
-                         A, n, j, V, taus, d, e, pin, pout);
+    const int len = n - j - 1, rows = n - j - 2;
+    const dim3 grid(rows > 0 ? (rows + 3) / 4 : 1);
+    if (len > eig::MAXN / 2)
+      hipLaunchKernelGGL(k_hetrd_step_wide, grid, dim3(eig::NT), lds2, s, A, n, j, V, taus, d, e, pin, pout);
+    else if (len > 768)
+      hipLaunchKernelGGL(k_hetrd_step<16>, grid, dim3(eig::NT), lds3, s, A, n, j, V, taus, d, e, pin, pout);
+    else if (len > 512)
+      hipLaunchKernelGGL(k_hetrd_step<12>, grid, dim3(eig::NT), lds3, s, A, n, j, V, taus, d, e, pin, pout);
+    else if (len > 256)
+      hipLaunchKernelGGL(k_hetrd_step<8>, grid, dim3(eig::NT), lds3, s, A, n, j, V, taus, d, e, pin, pout);
     else
-      hipLaunchKernelGGL(k_hetrd_step_wide, dim3(G), dim3(eig::NT), 2 * (size_t)n * sizeof(double2), s, A, n, j, V,
-                         taus, d, e, pin, pout);
+      hipLaunchKernelGGL(k_hetrd_step<4>, grid, dim3(eig::NT), lds3, s, A, n, j, V, taus, d, e, pin, pout);
   }
 }
 

@@ -519,7 +519,9 @@ def run_scf(system, settings: SCFSettings, *, density=None, wavefunctions=None, 
         for sp in range(nspin):
             psi = wf[sp].to(torch.complex128)
             dens2 = psi.real**2 + psi.imag**2
-            ekin += float((f_all[sp] * (dens2 @ kin[:, :, None])[..., 0]).sum().item())
+            # sum over plane waves as a reduction: as a matrix-vector product (K = npw, one output
+            # column) it ran on a handful of workgroups
+            ekin += float((f_all[sp] * (dens2 * kin[:, None, :]).sum(-1)).sum().item())
             if kb is not None:
                 mat, dij = kb[0], kb[1]
                 p = ops.zgemm(psi.contiguous(), mat, 1)
