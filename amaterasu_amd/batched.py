@@ -197,9 +197,12 @@ def _lowest(proj: torch.Tensor, used, nbands: int):
     return w, rot
 
 
-def _rayleigh_ritz(basis, h_basis, used, nbands):
-    m = _bgram(basis, h_basis)  # conj of the projected matrix
-    proj = _herm(m.conj())
+def _projected(basis, h_basis):
+    """The projected matrices <b_r|H|b_c> (Hermitian part, FP64) of whole blocks."""
+    return _herm(_bgram(basis, h_basis).conj())
+
+
+def _rayleigh_ritz(proj, basis, h_basis, used, nbands):
     w, rot = _lowest(proj, used, nbands)
     return w, _brot(rot, basis), _brot(rot, h_basis)
 
@@ -226,9 +229,14 @@ def solve_batched(apply_block, guess: torch.Tensor, kinetic: torch.Tensor, npw, 
     basis[:, :nb] = v0
     hbasis[:, :nb] = apply_block(v0)
     m = np.full(nk, nb)
-    eps, vec, hvec = _rayleigh_ritz(basis[:, :nb], hbasis[:, :nb], full, nb)
+    # the projected matrices are kept between steps: a step adds only the columns of its new
+    # directions (8 m n_new npw flops instead of 8 m^2 npw), the matrix the reference rebuilds
+    # from scratch (subspace.py:72) up to rounding; a collapse recomputes the Ritz block
+    proj = torch.zeros((nk, cap, cap), dtype=torch.complex128, device=dev)
+    proj[:, :nb, :nb] = _projected(basis[:, :nb], hbasis[:, :nb])
+    eps, vec, hvec = _rayleigh_ritz(proj[:, :nb, :nb], basis[:, :nb], hbasis[:, :nb], full, nb)
     # frozen outputs of the k-points that have returned
-    out_eps, out_vec = eps.clone(), vec.clone()
+    out_eps, out_vec = eps, vec
     out_norm = torch.full((nk, nb), float("inf"), dtype=torch.float64, device=dev)
     running = np.ones(nk, dtype=bool)
     iters = np.zeros(nk, dtype=int)
@@ -237,7 +245,10 @@ def solve_batched(apply_block, guess: torch.Tensor, kinetic: torch.Tensor, npw, 
         iters[running] = step
         res, norms = ops.residual(hvec.reshape(nk * nb, npad), vec.reshape(nk * nb, npad), eps.reshape(-1))
         norms = norms.reshape(nk, nb)
-        out_norm[torch.as_tensor(running, device=dev)] = norms[torch.as_tensor(running, device=dev)]
+        if running.all():
+            out_norm = norms
+        else:
+            out_norm[torch.as_tensor(running, device=dev)] = norms[torch.as_tensor(running, device=dev)]
         active = (norms > tolerance).cpu().numpy() & running[:, None]
         finished = running & ~active.any(axis=1)
         running &= ~finished
@@ -245,7 +256,9 @@ def solve_batched(apply_block, guess: torch.Tensor, kinetic: torch.Tensor, npw, 
             break
         nnew = active.sum(axis=1)
         width = int(nnew.max())
-        corr = torch.zeros((nk, width, npad), dtype=cdt, device=dev)
+        # rows past a k-point's own count stay zero (empty only when every k fills the width)
+        full_width = bool((nnew[running] == width).all()) and running.all()
+        corr = (torch.empty if full_width else torch.zeros)((nk, width, npad), dtype=cdt, device=dev)
         for k in np.flatnonzero(running):
             rows = torch.as_tensor(np.flatnonzero(active[k]) + k * nb, dtype=torch.int32, device=dev)
             corr[k, : nnew[k]] = ops.precondition(res, vec.reshape(nk * nb, npad), kin[k], rows)
@@ -257,10 +270,11 @@ def solve_batched(apply_block, guess: torch.Tensor, kinetic: torch.Tensor, npw, 
         rn = torch.linalg.vector_norm(corr, dim=-1)
         keep = (rn > 1e-7).cpu().numpy() & (np.arange(width)[None, :] < nnew[:, None])
         nkeep = keep.sum(axis=1)
-        # compact the kept rows to the front of each k's block
-        order = np.argsort(~keep, axis=1, kind="stable")
-        corr = torch.gather(corr, 1, torch.as_tensor(order, device=dev)[:, :, None].expand(-1, -1, npad))
-        corr = corr * torch.as_tensor(np.arange(width)[None, :] < nkeep[:, None], device=dev)[:, :, None]
+        if (nkeep != nnew).any():
+            # compact the kept rows to the front of each k's block and zero the dropped ones
+            order = np.argsort(~keep, axis=1, kind="stable")
+            corr = torch.gather(corr, 1, torch.as_tensor(order, device=dev)[:, :, None].expand(-1, -1, npad))
+            corr = corr * torch.as_tensor(np.arange(width)[None, :] < nkeep[:, None], device=dev)[:, :, None]
         new, kept_t = _gram_rows(corr.contiguous(), torch.as_tensor(nkeep, device=dev), 1e-7)
         kept = kept_t.cpu().numpy()
         stopped = running & (kept == 0)
@@ -268,24 +282,40 @@ def solve_batched(apply_block, guess: torch.Tensor, kinetic: torch.Tensor, npw, 
         if not running.any():
             break
         kept = np.where(running, kept, 0)
-        hnew = apply_block(new.contiguous())
+        new = new.contiguous()
+        hnew = apply_block(new)
         for k in np.flatnonzero(running):
             basis[k, m[k] : m[k] + kept[k]] = new[k, : kept[k]]
             hbasis[k, m[k] : m[k] + kept[k]] = hnew[k, : kept[k]]
+        cur = int((m + kept).max())
+        cols = _bgram(basis[:, :cur], hnew)  # conj of <b_r|H|new_j>, rows of new past kept are zero
+        for k in np.flatnonzero(running):
+            a, b = int(m[k]), int(m[k] + kept[k])
+            ck = cols[k, :b, : kept[k]].conj().to(torch.complex128)
+            proj[k, :a, a:b] = ck[:a]
+            proj[k, a:b, :a] = ck[:a].conj().T
+            proj[k, a:b, a:b] = _herm(ck[a:b])
         m = m + kept
-        cur = int(m.max())
-        eps_n, vec_n, hvec_n = _rayleigh_ritz(basis[:, :cur], hbasis[:, :cur], torch.as_tensor(m, device=dev), nb)
-        run_t = torch.as_tensor(running, device=dev)
-        eps = torch.where(run_t[:, None], eps_n, eps)
-        vec = torch.where(run_t[:, None, None], vec_n, vec)
-        hvec = torch.where(run_t[:, None, None], hvec_n, hvec)
-        out_eps = torch.where(run_t[:, None], eps, out_eps)
-        out_vec = torch.where(run_t[:, None, None], vec, out_vec)
+        eps_n, vec_n, hvec_n = _rayleigh_ritz(proj[:, :cur, :cur], basis[:, :cur], hbasis[:, :cur],
+                                              torch.as_tensor(m, device=dev), nb)
+        if running.all():
+            eps, vec, hvec = eps_n, vec_n, hvec_n
+            out_eps, out_vec = eps, vec
+        else:
+            run_t = torch.as_tensor(running, device=dev)
+            eps = torch.where(run_t[:, None], eps_n, eps)
+            vec = torch.where(run_t[:, None, None], vec_n, vec)
+            hvec = torch.where(run_t[:, None, None], hvec_n, hvec)
+            out_eps = torch.where(run_t[:, None], eps, out_eps)
+            out_vec = torch.where(run_t[:, None, None], vec, out_vec)
         for k in np.flatnonzero(running & (m + nb > np.asarray(maxsub))):
-            basis[k].zero_()
-            hbasis[k].zero_()
+            if nk > 1:  # rows nb .. m[k] take part in the other k-points' padded products
+                basis[k, nb : m[k]].zero_()
+                hbasis[k, nb : m[k]].zero_()
             basis[k, :nb] = vec[k]
             hbasis[k, :nb] = hvec[k]
+            proj[k].zero_()
+            proj[k, :nb, :nb] = _projected(basis[k : k + 1, :nb], hbasis[k : k + 1, :nb])[0]
             m[k] = nb
     conv = (out_norm <= tolerance).all(dim=1).cpu().numpy()
     return BatchedResult(out_eps, out_vec, out_norm, iters, conv)

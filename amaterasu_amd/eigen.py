@@ -77,17 +77,16 @@ def _hermitian(m: torch.Tensor) -> torch.Tensor:
 
 
 def _cholesky_orthonormalize(block: torch.Tensor):
-    """L^{-1} block with L L^H = block block^H; None if the Cholesky fails."""
+    """L^{-1} block with L L^H = block block^H; None if the Cholesky fails. One host
+    synchronisation: the factorisation's status and the finiteness of L^{-1} (a finite
+    L^{-1} times a finite block is finite) are read together."""
     s = _hermitian(_gram(block, block))
     lower, info = torch.linalg.cholesky_ex(s)
-    if int(info.item()) != 0:
-        return None
     eye = torch.eye(s.shape[0], dtype=s.dtype, device=s.device)
     linv = torch.linalg.solve_triangular(lower, eye, upper=False)
-    out = _rotate(linv, block)
-    if not bool(torch.isfinite(torch.view_as_real(out)).all()):
+    if not bool(((info == 0) & torch.isfinite(torch.view_as_real(linv)).all()).item()):
         return None
-    return out
+    return _rotate(linv, block)
 
 
 def _cholesky_keeps_all(g: torch.Tensor, threshold: float):
@@ -145,11 +144,17 @@ def orthonormalize(block):
     return out.cpu().numpy() if host else out
 
 
-def orthogonalize_against(block, basis, *, threshold: float = 1e-7):
+def orthogonalize_against(block, basis, *, threshold: float = 1e-7, overwrite: bool = False):
     """``subspace.py:41-60``: two projections off ``basis``, drop rows with norm <= threshold,
-    orthonormalise keeping singular values > threshold * s_max."""
+    orthonormalise keeping singular values > threshold * s_max. ``overwrite=True`` lets the
+    projections work in a contiguous device ``block`` in place (the caller's temporary)."""
     host = not isinstance(block, torch.Tensor)
-    b = _to_device(block) if host else block.contiguous().clone()
+    if host:
+        b = _to_device(block)
+    else:
+        b = block.contiguous()  # a copy already when block was not contiguous
+        if not overwrite and b.data_ptr() == block.data_ptr():
+            b = b.clone()
     q = _to_device(basis, b.dtype) if host else basis.contiguous()
     for _ in range(2):
         if q.shape[0] and b.shape[0]:
@@ -157,7 +162,8 @@ def orthogonalize_against(block, basis, *, threshold: float = 1e-7):
             ops.zgemm(c, q, 0, alpha=-1.0, beta=1.0, out=b)
     norms = torch.linalg.vector_norm(b, dim=1)
     keep = torch.nonzero(norms > threshold).flatten()
-    b = b.index_select(0, keep).contiguous()
+    if keep.numel() != b.shape[0]:
+        b = b.index_select(0, keep).contiguous()
     if b.shape[0] == 0:
         return b.cpu().numpy() if host else b
     out = _gram_orthonormalize(b, threshold)
@@ -266,7 +272,7 @@ def davidson(apply_hamiltonian, guess, kinetic, *, tolerance: float = 1e-8, max_
         if active.numel() == 0:
             break
         corr = ops.precondition(res, vec, kin, active)
-        new = orthogonalize_against(corr, basis[:m])
+        new = orthogonalize_against(corr, basis[:m], overwrite=True)
         nn = new.shape[0]
         if nn == 0:
             break
