@@ -40,19 +40,29 @@ def enabled() -> bool:
     return os.environ.get("AMT_AUTOTUNE", "0") == "1"
 
 
-def heuristic_split(m: int, n: int, k: int, batch: int, prec: int = 0) -> int:
-    """The library's built-in choice (``choose_split`` in gemm.hip): the smallest split giving
-    about 1024 workgroups of 64 x 64 tiles and, in FP32 (prec 0), K chunks of at most 1024;
-    then as many splits as fit in the same number of 1024-workgroup rounds; at least 256 of k
-    per split, at most 256 splits."""
-    tiles = ((m + 63) // 64) * ((n + 63) // 64) * batch
-    if tiles >= 512:
-        return 1
-    single = (k + 1023) // 1024 if prec == 0 else 1
-    want = max((1024 + tiles - 1) // tiles, single)
-    rounds = max(1, math.ceil(tiles * want / 1024 - 0.05))
-    want = max(rounds * 1024 // tiles, single)
-    return int(max(1, min(want, (k + 255) // 256, 256)))
+def heuristic_split(m: int, n: int, k: int, batch: int, prec: int = 0, op: int = 1) -> int:
+    """The library's built-in choice (``plan_gemm`` / ``choose_split`` in gemm.hip): the tile
+    (80 rows where that pads less, for FP32 NN, FP32 NC with m < 256 and FP64 NN, unless the
+    FP32 chunks would then need the two-level kernel; else 64 rows; 64 columns), then the
+    smallest split giving about 1024 workgroups and, in FP32 (prec 0), K chunks of at most
+    1024; then as many splits as fit in the same number of 1024-workgroup rounds; at least 256
+    of k per split, at most 256 splits."""
+
+    def split(rows):
+        tiles = ((m + rows - 1) // rows) * ((n + 63) // 64) * batch
+        if tiles >= 512:
+            return 1
+        single = (k + 1023) // 1024 if prec == 0 else 1
+        want = max((1024 + tiles - 1) // tiles, single)
+        rounds = max(1, math.ceil(tiles * want / 1024 - 0.05))
+        want = max(rounds * 1024 // tiles, single)
+        return int(max(1, min(want, (k + 255) // 256, 256)))
+
+    tall = (m + 79) // 80 * 80 < (m + 63) // 64 * 64 and (op == 0 or (prec == 0 and m < 256))
+    ns = split(80 if tall else 64)
+    if tall and prec == 0 and (k + ns - 1) // ns > 1024:
+        ns = split(64)
+    return ns
 
 
 def bucket(prec: int, op: int, m: int, n: int, k: int, batch: int) -> tuple:
@@ -74,7 +84,7 @@ def tune_zgemm(run, prec: int, op: int, m: int, n: int, k: int, batch: int) -> i
         _TUNED[key] = 0
         return 0
     kmax = max(1, (k + 255) // 256)
-    h = heuristic_split(m, n, k, batch, prec)
+    h = heuristic_split(m, n, k, batch, prec, op)
     if h == 1 and kmax == 1:
         _TUNED[key] = 0
         return 0

@@ -66,9 +66,19 @@ struct Mfma<double> {
 constexpr int BM = 64, BN = 64, BK = 16;
 constexpr int kFlushTiles = 4;
 
-// TM = 1: 64 x 64 tiles (each wavefront 32 x 32 as 2 x 2 MFMA blocks); TM = 2 (FP32, single
-// level): 128 x 128 tiles, each wavefront 64 x 64 as 4 x 4 blocks, half the operand traffic
-// per MFMA for large products.
+// Tile shapes TM: 1 = 64 x 64 tiles (each wavefront 32 x 32 as 2 x 2 MFMA blocks); 2 (FP32,
+// single level) = 128 x 128 tiles, each wavefront 64 x 64 as 4 x 4 blocks, half the operand
+// traffic per MFMA for large products; kTall = 80 x 64 tiles, the four wavefronts side by
+// side, each 80 x 16 as 5 x 1 blocks: 160-, 480- or 800-row operands (the Si64 band blocks
+// and their multiples) fill them exactly, where 64-row tiles compute 20 % of padding.
+constexpr int kTall = 5;
+
+template <int TM>
+struct TileShape {
+  static constexpr int rows = TM == kTall ? 80 : BM * TM, cols = TM == kTall ? 64 : BN * TM;
+  static constexpr int wave_rows = TM == kTall ? 5 : 2 * TM, wave_cols = TM == kTall ? 1 : 2 * TM;  // MFMA blocks
+};
+
 template <typename R, int OP, bool kTwoLevel, int TM>
 __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, int K, int kchunk, int nsplit,
                                                const typename Cplx<R>::T* __restrict__ A, long long lda,
@@ -76,11 +86,13 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
                                                long long ldb, long long sB, typename Cplx<R>::T* __restrict__ Cm,
                                                long long ldc, long long sC, R alpha, R beta,
                                                typename Cplx<R>::T* __restrict__ part, int mfast) {
-  static_assert(TM == 1 || !kTwoLevel, "the two-level accumulation uses 64 x 64 tiles");
+  static_assert(TM != 2 || !kTwoLevel, "the two-level accumulation uses the 64- or 80-row tiles");
   using C = typename Cplx<R>::T;
   using acc_t = typename Mfma<R>::acc_t;
-  constexpr int TBM = BM * TM, TBN = BN * TM, WB = 2 * TM;  // tile, MFMA blocks per wave side
-  constexpr int KQ = 4 * TM;                                 // consecutive k per thread in a row load
+  constexpr int TBM = TileShape<TM>::rows, TBN = TileShape<TM>::cols;
+  constexpr int WBR = TileShape<TM>::wave_rows, WBC = TileShape<TM>::wave_cols;  // MFMA blocks per wave
+  constexpr int KQ = TM == kTall ? 4 : 4 * TM;  // consecutive k (or columns) per thread in a 64-wide load
+  constexpr int AE = TBM * BK / 256;            // A elements per thread and K tile
   __shared__ C As[2][BK][TBM + 1];  // double-buffered K tiles
   __shared__ C Bs[2][BK][TBN + 1];
   // XCD-aware order: workgroups are handed to the 8 XCDs round-robin by linear id, so
@@ -100,12 +112,12 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
   A += (long long)batch * sA;
   B += (long long)batch * sB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 1) * 32 * TM, wc = (wave & 1) * 32 * TM;
-  acc_t p1[WB][WB], p2[WB][WB], p3[WB][WB];  // Gauss's three products
+  const int wr = TM == kTall ? 0 : (wave >> 1) * 32 * TM, wc = TM == kTall ? wave * 16 : (wave & 1) * 32 * TM;
+  acc_t p1[WBR][WBC], p2[WBR][WBC], p3[WBR][WBC];  // Gauss's three products
 #pragma unroll
-  for (int a = 0; a < WB; ++a)
+  for (int a = 0; a < WBR; ++a)
 #pragma unroll
-    for (int b = 0; b < WB; ++b)
+    for (int b = 0; b < WBC; ++b)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         p1[a][b][q] = 0;
@@ -113,13 +125,13 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
         p3[a][b][q] = 0;
       }
   // second-level FP64 sums of the FP32 accumulators (two-level form only)
-  double sumr[kTwoLevel ? 2 : 1][kTwoLevel ? 2 : 1][4], sumi[kTwoLevel ? 2 : 1][kTwoLevel ? 2 : 1][4];
+  double sumr[kTwoLevel ? WBR : 1][kTwoLevel ? WBC : 1][4], sumi[kTwoLevel ? WBR : 1][kTwoLevel ? WBC : 1][4];
   auto flush = [&]() {
     if constexpr (kTwoLevel) {
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < WBR; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < WBC; ++b)
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const double t1 = p1[a][b][q], t2 = p2[a][b][q], t3 = p3[a][b][q];
@@ -133,9 +145,9 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
   };
   if constexpr (kTwoLevel) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < WBR; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < WBC; ++b)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           sumr[a][b][q] = 0.0;
@@ -144,9 +156,16 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
   }
   // Global -> register loads of one K tile (bounds-checked, conj(B) for NC), and the store of
   // those registers into an LDS buffer: A k-major As[k][row]; B k-major Bs[k][col].
-  C ra[KQ], rb[KQ];
+  // The 80-row tile loads A as AE single elements per thread (16 threads per row of 16 k).
+  C ra[TM == kTall ? AE : KQ], rb[KQ];
   auto gload = [&](int k0) {
-    {
+    if constexpr (TM == kTall) {
+#pragma unroll
+      for (int u = 0; u < AE; ++u) {
+        const int idx = tid + 256 * u, row = idx / BK, gm = m0 + row, gk = k0 + idx % BK;
+        ra[u] = (gm < M && gk < kend) ? A[(long long)gm * lda + gk] : cmake<C>(0, 0);
+      }
+    } else {
       const int row = tid / (BK / KQ), kq = (tid % (BK / KQ)) * KQ;
       const int gm = m0 + row;
 #pragma unroll
@@ -176,7 +195,13 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
     }
   };
   auto sstore = [&](int buf) {
-    {
+    if constexpr (TM == kTall) {
+#pragma unroll
+      for (int u = 0; u < AE; ++u) {
+        const int idx = tid + 256 * u;
+        As[buf][idx % BK][idx / BK] = ra[u];
+      }
+    } else {
       const int row = tid / (BK / KQ), kq = (tid % (BK / KQ)) * KQ;
 #pragma unroll
       for (int u = 0; u < KQ; ++u) As[buf][kq + u][row] = ra[u];
@@ -195,15 +220,15 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       const int kr = ks * 4 + (lane >> 4);
-      C a[WB], b[WB];
+      C a[WBR], b[WBC];
 #pragma unroll
-      for (int rb2 = 0; rb2 < WB; ++rb2) a[rb2] = As[buf][kr][wr + rb2 * 16 + (lane & 15)];
+      for (int rb2 = 0; rb2 < WBR; ++rb2) a[rb2] = As[buf][kr][wr + rb2 * 16 + (lane & 15)];
 #pragma unroll
-      for (int cb = 0; cb < WB; ++cb) b[cb] = Bs[buf][kr][wc + cb * 16 + (lane & 15)];
+      for (int cb = 0; cb < WBC; ++cb) b[cb] = Bs[buf][kr][wc + cb * 16 + (lane & 15)];
 #pragma unroll
-      for (int rb2 = 0; rb2 < WB; ++rb2)
+      for (int rb2 = 0; rb2 < WBR; ++rb2)
 #pragma unroll
-        for (int cb = 0; cb < WB; ++cb) {
+        for (int cb = 0; cb < WBC; ++cb) {
           p1[rb2][cb] = Mfma<R>::mma(a[rb2].x, b[cb].x, p1[rb2][cb]);
           p2[rb2][cb] = Mfma<R>::mma(a[rb2].y, b[cb].y, p2[rb2][cb]);
           p3[rb2][cb] = Mfma<R>::mma(a[rb2].x + a[rb2].y, b[cb].x + b[cb].y, p3[rb2][cb]);
@@ -235,9 +260,9 @@ __global__ __launch_bounds__(256, TM == 2 ? 2 : 1) void k_zgemm(int M, int N, in
   }
   flush();
 #pragma unroll
-  for (int rb = 0; rb < WB; ++rb)
+  for (int rb = 0; rb < WBR; ++rb)
 #pragma unroll
-    for (int cb = 0; cb < WB; ++cb)
+    for (int cb = 0; cb < WBC; ++cb)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = m0 + wr + rb * 16 + Mfma<R>::row(lane, q);
@@ -326,7 +351,7 @@ static std::array<int, 6> split_key(int prec, int op, int m, int n, int k, int b
 
 constexpr int kLongChunk = 1024;
 
-static int choose_split(int prec, int op, int m, int n, int k, int batch) {
+static int choose_split(int prec, int op, int m, int n, int k, int batch, int tile_rows) {
   const long long kmax = (k + 255) / 256;
   {
     std::lock_guard<std::mutex> lock(g_split_mutex);
@@ -337,7 +362,7 @@ static int choose_split(int prec, int op, int m, int n, int k, int batch) {
       return s < 1 ? 1 : (int)s;
     }
   }
-  const long long tiles = (long long)((m + BM - 1) / BM) * ((n + BN - 1) / BN) * batch;
+  const long long tiles = (long long)((m + tile_rows - 1) / tile_rows) * ((n + BN - 1) / BN) * batch;
   if (tiles >= 512) return 1;
   long long want = (1024 + tiles - 1) / tiles;
   const long long single = prec == kFP32 ? (k + kLongChunk - 1) / kLongChunk : 1;
@@ -381,7 +406,7 @@ constexpr int kBigChunk = 512;
 constexpr size_t kBigWorkspace = size_t(24) << 30;
 
 struct GemmPlan {
-  int tm;  // tile multiplier
+  int tm;  // tile shape (1, 2 or kTall)
   int ns;  // K splits (before rounding the chunk to BK)
 };
 
@@ -398,7 +423,17 @@ static GemmPlan plan_gemm(int prec, int op, int m, int n, int k, int batch) {
     const size_t bytes = ns > 1 ? (size_t)batch * ns * m * n * 8 : 0;
     if (batch * ns <= 65535 && bytes <= kBigWorkspace) return {2, (int)ns};
   }
-  return {1, choose_split(prec, op, m, n, k, batch)};
+  // 80-row tiles where they leave fewer padded rows than 64-row ones (m = 160, 480, ...), for
+  // the products where they measured faster (Si64 shapes, 160-row blocks): FP32 NN and NC with
+  // m < 256 on the single-level kernel (10-21 %), FP64 NN (3-5 %). Their two-level FP32 form
+  // and FP64 NC products run at one wave per SIMD and were slower.
+  bool tall = (m + 79) / 80 * 80 < (m + 63) / 64 * 64 && (prec == kFP32 ? (op == 0 || m < 256) : op == 0);
+  int ns = choose_split(prec, op, m, n, k, batch, tall ? TileShape<kTall>::rows : TileShape<1>::rows);
+  if (tall && prec == kFP32 && (k + ns - 1) / ns > kLongChunk) {
+    tall = false;
+    ns = choose_split(prec, op, m, n, k, batch, TileShape<1>::rows);
+  }
+  return {tall ? kTall : 1, ns};
 }
 
 static bool takes_large_tiles(int prec, int op, int m, int n, int k, int batch) {
@@ -418,9 +453,9 @@ static void launch_tile(int op, dim3 grid, hipStream_t s, int m, int n, int k, i
                         long long ldc, long long sc, double alpha, double beta, typename Cplx<R>::T* part) {
   using C = typename Cplx<R>::T;
   const int mfast = op == 0 && grid.y > 1 && grid.x >= grid.y;
-  if constexpr (TM == 1) {
+  if constexpr (TM == 1 || TM == kTall) {
     if (op == 0) {
-      hipLaunchKernelGGL((k_zgemm<R, 0, kTwoLevel, 1>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a,
+      hipLaunchKernelGGL((k_zgemm<R, 0, kTwoLevel, TM>), grid, dim3(256), 0, s, m, n, k, kchunk, ns, (const C*)a,
                          lda, sa, (const C*)b, ldb, sb, (C*)c, ldc, sc, (R)alpha, (R)beta, part, mfast);
       return;
     }
@@ -458,19 +493,23 @@ static int zgemm_t(int op, int m, int n, int k, double alpha, const void* a, lon
     set_error("zgemm: batch x split exceeds the grid limit");
     return 1;
   }
-  const int tb = g.tm == 2 ? 2 * BM : BM;
-  const dim3 grid((n + tb - 1) / tb, (m + tb - 1) / tb, batch * ns);
+  const int trows = g.tm == 2 ? TileShape<2>::rows : g.tm == kTall ? TileShape<kTall>::rows : TileShape<1>::rows;
+  const int tcols = g.tm == 2 ? TileShape<2>::cols : g.tm == kTall ? TileShape<kTall>::cols : TileShape<1>::cols;
+  const dim3 grid((n + tcols - 1) / tcols, (m + trows - 1) / trows, batch * ns);
   C* const p = ns > 1 ? part : nullptr;
+#define AMT_TILE(TWO, SHAPE) \
+  launch_tile<R, TWO, SHAPE>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p)
   if constexpr (std::is_same<R, float>::value) {
     if (g.tm == 2)
-      launch_tile<R, false, 2>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
+      AMT_TILE(false, 2);
     else if (kchunk > kLongChunk)
-      launch_tile<R, true, 1>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
+      g.tm == kTall ? AMT_TILE(true, kTall) : AMT_TILE(true, 1);
     else
-      launch_tile<R, false, 1>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
+      g.tm == kTall ? AMT_TILE(false, kTall) : AMT_TILE(false, 1);
   } else {
-    launch_tile<R, false, 1>(op, grid, s, m, n, k, kchunk, ns, a, lda, sa, b, ldb, sb, c, ldc, sc, alpha, beta, p);
+    g.tm == kTall ? AMT_TILE(false, kTall) : AMT_TILE(false, 1);
   }
+#undef AMT_TILE
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error(std::string("zgemm launch: ") + hipGetErrorString(e));

@@ -16,8 +16,11 @@ def _c(shape, seed):
 @pytest.mark.parametrize("dtype,tol", [("complex128", 1e-13), ("complex64", 2e-5)])
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (17, 33, 9), (64, 64, 16), (100, 70, 300), (8, 36, 754),
                                    (160, 1152, 3000), (37, 5, 40000), (300, 270, 1000), (257, 513, 700),
-                                   (256, 384, 3000)])
+                                   (256, 384, 3000), (150, 90, 700), (470, 33, 5000), (150, 20000, 1100)])
 def test_zgemm_nn_and_nc(dtype, tol, m, n, k):
+    """Every tile shape: 64- and 128-row tiles, 80-row tiles with ragged edges (m = 150, 470),
+    and an 80-row candidate whose single K chunk would need the two-level kernel (m = 150,
+    k = 1100 with 626 output tiles: it falls back to 64-row tiles)."""
     import torch
 
     from amaterasu_amd import ops
