@@ -261,7 +261,8 @@ def solve_batched(apply_block, guess: torch.Tensor, kinetic: torch.Tensor, npw, 
         corr = (torch.empty if full_width else torch.zeros)((nk, width, npad), dtype=cdt, device=dev)
         for k in np.flatnonzero(running):
             rows = torch.as_tensor(np.flatnonzero(active[k]) + k * nb, dtype=torch.int32, device=dev)
-            corr[k, : nnew[k]] = ops.precondition(res, vec.reshape(nk * nb, npad), kin[k], rows)
+            corr[k, : nnew[k]] = ops.precondition(res, vec.reshape(nk * nb, npad), kin[k], rows,
+                                                  rows_checked=True)  # host-built from this k's bands
         # orthogonalise against the current basis (its padded rows are zero), twice
         cur = int(m.max())
         for _ in range(2):

@@ -184,8 +184,8 @@ __device__ __forceinline__ double2 block_sum2_once(double2 v, double* red) {
 // sum p^H v, the publication of w, and the norm of the new column. The reflector of column
 // j+1 (zlarfg) is kept unscaled in LDS; its scale enters the product with the owned row
 // once, after the row's dot product, and the stored V row.
-// kRowRegs = elements of one row per lane, 64 kRowRegs >= the trailing length n - j - 1 (4, 8, 12
-// or 16; longer rows: k_hetrd_step_wide).
+// kRowRegs = elements of one row per lane, 64 kRowRegs >= the trailing length n - j - 1 (8, 12
+// or 16; longer rows: k_hetrd_step_wide, shorter ones: k_hetrd_step_w64).
 
 template <int kRowRegs>
 __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A, int n, int j,
@@ -331,6 +331,138 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A,
       acc.x += __shfl_down(acc.x, off, 64);
       acc.y += __shfl_down(acc.y, off, 64);
     }
+    if (lane == 0) p_out[myrow - j - 2] = zmul(tau2, acc);
+  }
+}
+
+// Steps whose trailing length is at most 64 Q, one wavefront (one trailing row) per
+// workgroup: lane l holds the elements t = l + 64 q (q < Q) of v_j, p_j, w_j and of the pivot
+// row, and the same columns of its row, so the row update and the product with v_{j+1} use
+// the lane's own elements only and the three sums are wavefront reductions: no LDS and no
+// barrier. Same formulas as k_hetrd_step.
+constexpr int kW64Max = 8;  // Q of the longest wavefront step (trailing length 512)
+
+__device__ __forceinline__ double2 wave_sum2(double2 v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_xor(v.x, off, 64);
+    v.y += __shfl_xor(v.y, off, 64);
+  }
+  return v;
+}
+
+// Element idx (uniform across the wavefront) of a vector distributed as t = lane + 64 q; the
+// element is selected from the registers by unrolled compares (no dynamic register index).
+template <int Q>
+__device__ __forceinline__ double2 w64_element(const double2 (&x)[Q], int idx) {
+  const int q = idx >> 6, src = idx & 63;
+  double2 v = x[0];
+#pragma unroll
+  for (int u = 1; u < Q; ++u)
+    if (q == u) v = x[u];
+  return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+#define AMT_W64(x, idx) w64_element<Q>(x, idx)
+
+template <int Q>
+__global__ __launch_bounds__(64) void k_hetrd_step_w64(double2* __restrict__ A, int n, int j,
+                                                       double2* __restrict__ V, double2* __restrict__ taus,
+                                                       double* __restrict__ d, double* __restrict__ e,
+                                                       const double2* __restrict__ p_in, double2* __restrict__ p_out) {
+  const int len = n - j - 1;
+  const int lane = threadIdx.x;
+  const int myrow = j + 2 + blockIdx.x;
+  const bool own = myrow < n;
+  const double2 zero = make_double2(0.0, 0.0);
+  double2 rowv[Q], piv[Q], vq[Q], pq[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int t = lane + 64 * q, c = j + 1 + t;
+    rowv[q] = (own && c < n) ? A[(long long)myrow * n + c] : make_double2(0.0, 0.0);
+    piv[q] = t < len ? A[(long long)(j + 1) * n + c] : make_double2(0.0, 0.0);
+    vq[q] = t < len ? V[(long long)j * n + t] : make_double2(0.0, 0.0);
+    pq[q] = t < len ? p_in[t] : make_double2(0.0, 0.0);
+  }
+  const double2 tau = taus[j];
+  // K = -(tau/2) p^H v; w = p + K v
+  double2 part = zero;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) part = cadd(part, zmul(zconj(pq[q]), vq[q]));
+  const double2 K = zmul(make_double2(-0.5 * tau.x, -0.5 * tau.y), wave_sum2(part));
+  double2 wq[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) wq[q] = cadd(pq[q], zmul(K, vq[q]));
+  // the owned row: A -= v w^H + w v^H
+  if (own) {
+    const int tr = myrow - j - 1;
+    const double2 vr = AMT_W64(vq, tr), wr = AMT_W64(wq, tr);
+    double2* row = A + (long long)myrow * n;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int c = j + 1 + lane + 64 * q;
+      if (c < n) {
+        rowv[q] = csub(rowv[q], cadd(zmul(vr, zconj(wq[q])), zmul(wr, zconj(vq[q]))));
+        row[c] = rowv[q];
+      }
+    }
+  }
+  const double2 v0 = AMT_W64(vq, 0), w0 = AMT_W64(wq, 0);
+  const int len2 = n - j - 2;
+  if (len2 <= 0) {
+    if (blockIdx.x == 0 && lane == 0) d[j + 1] = piv[0].x - 2.0 * zmul(v0, zconj(w0)).x;
+    return;
+  }
+  // the updated pivot row j+1 and the column below the diagonal (its conjugate, t >= 1)
+  double2 xq[Q];
+  double nrm2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int t = lane + 64 * q;
+    const double2 nr = csub(piv[q], cadd(zmul(v0, zconj(wq[q])), zmul(w0, zconj(vq[q]))));
+    xq[q] = zconj(nr);
+    if (t == 0 && blockIdx.x == 0) d[j + 1] = nr.x;
+    if (t >= 2 && t < len) nrm2 += nr.x * nr.x + nr.y * nr.y;
+  }
+  const double xnorm2 = wave_sum2(make_double2(nrm2, 0.0)).x;
+  const double2 alpha = AMT_W64(xq, 1);
+  double beta;
+  double2 tau2, sc;
+  if (xnorm2 == 0.0 && alpha.y == 0.0) {
+    beta = alpha.x;
+    tau2 = zero;
+    sc = zero;
+  } else {
+    const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
+    beta = alpha.x >= 0.0 ? -nrm : nrm;
+    tau2 = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+    const double2 dd = make_double2(alpha.x - beta, alpha.y);
+    const double den = dd.x * dd.x + dd.y * dd.y;
+    sc = make_double2(dd.x / den, -dd.y / den);
+  }
+  if (blockIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int t = lane + 64 * q;
+      if (t >= 1 && t < len) V[(long long)(j + 1) * n + t - 1] = t == 1 ? make_double2(1.0, 0.0) : zmul(xq[q], sc);
+    }
+    if (lane == 0) {
+      taus[j + 1] = tau2;
+      e[j + 1] = beta;
+    }
+  }
+  // p_{j+1} for the owned row: tau2 (A[r][j+2] + sc sum_{c >= j+3} A[r][c] x[c - j - 2]); column
+  // c = j + 1 + t pairs with x at t - 1, which is this lane's xq
+  if (own) {
+    double2 a0 = zero, a1 = zero;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int t = lane + 64 * q, c = j + 1 + t;
+      if (t == 1) {
+        a0 = rowv[q];
+      } else if (t >= 2 && c < n) {
+        a1 = cadd(a1, zmul(rowv[q], xq[q]));
+      }
+    }
+    const double2 acc = wave_sum2(cadd(a0, zmul(sc, a1)));
     if (lane == 0) p_out[myrow - j - 2] = zmul(tau2, acc);
   }
 }
@@ -1218,10 +1350,14 @@ static void enqueue_tridiag(hipStream_t s, int n, double2* A, double2* V, double
       hipLaunchKernelGGL(k_hetrd_step<16>, grid, dim3(eig::NT), lds3, s, A, n, j, V, taus, d, e, pin, pout);
     else if (len > 512)
       hipLaunchKernelGGL(k_hetrd_step<12>, grid, dim3(eig::NT), lds3, s, A, n, j, V, taus, d, e, pin, pout);
-    else if (len > 256)
+    else if (len > 64 * kW64Max)
       hipLaunchKernelGGL(k_hetrd_step<8>, grid, dim3(eig::NT), lds3, s, A, n, j, V, taus, d, e, pin, pout);
+    else if (len > 256)
+      hipLaunchKernelGGL(k_hetrd_step_w64<kW64Max>, dim3(rows > 0 ? rows : 1), dim3(64), 0, s, A, n, j, V, taus, d,
+                         e, pin, pout);
     else
-      hipLaunchKernelGGL(k_hetrd_step<4>, grid, dim3(eig::NT), lds3, s, A, n, j, V, taus, d, e, pin, pout);
+      hipLaunchKernelGGL(k_hetrd_step_w64<4>, dim3(rows > 0 ? rows : 1), dim3(64), 0, s, A, n, j, V, taus, d, e,
+                         pin, pout);
   }
 }
 

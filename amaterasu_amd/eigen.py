@@ -271,7 +271,7 @@ def davidson(apply_hamiltonian, guess, kinetic, *, tolerance: float = 1e-8, max_
         active = torch.nonzero(norms > tolerance).flatten()
         if active.numel() == 0:
             break
-        corr = ops.precondition(res, vec, kin, active)
+        corr = ops.precondition(res, vec, kin, active, rows_checked=True)  # rows of nonzero(norms)
         new = orthogonalize_against(corr, basis[:m], overwrite=True)
         nn = new.shape[0]
         if nn == 0:

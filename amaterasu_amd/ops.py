@@ -177,8 +177,11 @@ def residual(hv: torch.Tensor, v: torch.Tensor, eps: torch.Tensor):
     return res, norms
 
 
-def precondition(res: torch.Tensor, v: torch.Tensor, kinetic: torch.Tensor, rows: torch.Tensor) -> torch.Tensor:
-    """Teter-Payne-Allan preconditioned, normalised residuals of the listed rows."""
+def precondition(res: torch.Tensor, v: torch.Tensor, kinetic: torch.Tensor, rows: torch.Tensor, *,
+                 rows_checked: bool = False) -> torch.Tensor:
+    """Teter-Payne-Allan preconditioned, normalised residuals of the listed rows.
+    ``rows_checked=True`` skips the range check of ``rows`` (two device reads), for callers
+    whose rows come from indices of ``v`` itself (``nonzero`` of a per-row mask)."""
     _require(res, "res")
     _require(v, "v")
     if res.shape != v.shape or res.dtype != v.dtype or v.dim() != 2:
@@ -188,8 +191,10 @@ def precondition(res: torch.Tensor, v: torch.Tensor, kinetic: torch.Tensor, rows
     if kin.numel() != v.shape[1]:
         raise AmtError("precondition: kinetic must have one entry per plane wave")
     r = rows.to(device=v.device, dtype=torch.int32).contiguous()
-    if r.numel() and (int(r.min().item()) < 0 or int(r.max().item()) >= v.shape[0]):
-        raise AmtError("precondition: row index out of range")
+    if r.numel() and not rows_checked:
+        lo, hi = torch.stack(torch.aminmax(r)).tolist()  # one device read
+        if lo < 0 or hi >= v.shape[0]:
+            raise AmtError("precondition: row index out of range")
     nout = int(r.numel())
     out = torch.empty((nout, v.shape[1]), dtype=v.dtype, device=v.device)
     check(lib().amt_precondition(prec, res.data_ptr(), v.data_ptr(), kin.data_ptr(), r.data_ptr(),
