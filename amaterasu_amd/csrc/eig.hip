@@ -16,7 +16,9 @@
 //       recurrence of LAPACK's dstebz), so each step narrows the bracket 513-fold;
 //    b. eigenvectors by inverse iteration with each eigenvalue as its own shift (tridiagonal
 //       LU with partial pivoting, pivots floored at eps ||T||, as LAPACK's dlagtf/dlagts),
-//       three rounds from pseudo-random starts. Eigenvalues closer than kClusterGap ||T||
+//       two rounds from pseudo-random starts (with a shift within 2 eps ||T|| of the
+//       eigenvalue, one round shrinks every other component by eps ||T|| / gap, below 1e-12
+//       for gaps above the cluster threshold). Eigenvalues closer than kClusterGap ||T||
 //       to a neighbour form a cluster, solved by one workgroup (a chain longer than
 //       kClusterMax is cut at its largest gaps into pieces that the workgroup solves in
 //       order, each orthogonalised against the piece before): after every round the
@@ -39,7 +41,7 @@ constexpr int NT = 256;
 constexpr int MAXN = 2048;
 constexpr int kClusterMax = 64;      // largest cluster solved together
 constexpr double kClusterGap = 2e-4;  // cluster threshold, relative to the Gershgorin bound of ||T||
-constexpr int kInverseRounds = 3;
+constexpr int kInverseRounds = 2;
 }  // namespace eig
 
 // Steps of a serial tridiagonal chain whose independent LDS loads are issued together (the
