@@ -44,6 +44,9 @@ constexpr double kClusterGap = 2e-4;  // cluster threshold, relative to the Gers
 constexpr int kInverseRounds = 2;
 }  // namespace eig
 
+// Columns per LDS slice of the Rayleigh-Ritz products inside a cluster.
+constexpr int kRRCols = 64;
+
 // Steps of a serial tridiagonal chain whose independent LDS loads are issued together (the
 // chain itself stays in registers, so LDS latency is paid once per chunk).
 constexpr int kTriChunk = 8;
@@ -1185,30 +1188,60 @@ __device__ void solve_piece(const double* __restrict__ d, const double* __restri
     return;
   }
   double* P = X + (long long)off * n;  // the piece's own rows
-  // Rayleigh-Ritz inside the piece: K = P T P^T (c x c), one entry pair per wave step
+  // Rayleigh-Ritz inside the piece: K = P (T P^T) (c x c). The rows of T P^T go to tmp (free
+  // until the rotation below); then slices of kRRCols columns of P and T P^T are staged in
+  // LDS and every thread accumulates its own entries (a <= b) over each slice.
+  for (int idx = tid; idx < c * n; idx += eig::NT) {
+    const int t = idx / n, i = idx - t * n;
+    const double* xb = P + (long long)t * n;
+    double y = d[i] * xb[i];
+    if (i > 0) y += e[i - 1] * xb[i - 1];
+    if (i + 1 < n) y += e[i] * xb[i + 1];
+    tmp[idx] = y;
+  }
+  constexpr int PPT = (eig::kClusterMax * (eig::kClusterMax + 1) / 2 + eig::NT - 1) / eig::NT;
   const int npair = c * (c + 1) / 2;
-  for (int pidx = wave; pidx < npair; pidx += eig::NT / 64) {
-    int a = 0, rem = pidx;
-    while (rem >= c - a) {
+  int pa[PPT], pb[PPT];
+  double kacc[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    int a = 0, rem = tid + eig::NT * q;
+    while (a < c && rem >= c - a) {
       rem -= c - a;
       ++a;
     }
-    const int b = a + rem;
-    const double* xa = P + (long long)a * n;
-    const double* xb = P + (long long)b * n;
-    double s = 0.0;
-    for (int i = lane; i < n; i += 64) {
-      double tb = d[i] * xb[i];
-      if (i > 0) tb += e[i - 1] * xb[i - 1];
-      if (i + 1 < n) tb += e[i] * xb[i + 1];
-      s += xa[i] * tb;
+    pa[q] = a;
+    pb[q] = a + rem;
+    kacc[q] = 0.0;
+  }
+  double* xs = tri_dyn;                               // [kClusterMax][kRRCols + 1]
+  double* ys = tri_dyn + eig::kClusterMax * (kRRCols + 1);
+  for (int i0 = 0; i0 < n; i0 += kRRCols) {
+    __syncthreads();
+    for (int idx = tid; idx < c * kRRCols; idx += eig::NT) {
+      const int t = idx / kRRCols, u = idx - t * kRRCols, i = i0 + u;
+      xs[t * (kRRCols + 1) + u] = i < n ? P[(long long)t * n + i] : 0.0;
+      ys[t * (kRRCols + 1) + u] = i < n ? tmp[(long long)t * n + i] : 0.0;
     }
-    s = wave_sum(s);
-    if (lane == 0) {
-      sh.K[a][b] = s;
-      sh.K[b][a] = s;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      if (tid + eig::NT * q < npair) {
+        const double* xa = xs + pa[q] * (kRRCols + 1);
+        const double* yb = ys + pb[q] * (kRRCols + 1);
+        double acc = 0.0;
+#pragma unroll 8
+        for (int u = 0; u < kRRCols; ++u) acc += xa[u] * yb[u];
+        kacc[q] += acc;
+      }
     }
   }
+#pragma unroll
+  for (int q = 0; q < PPT; ++q)
+    if (tid + eig::NT * q < npair) {
+      sh.K[pa[q]][pb[q]] = kacc[q];
+      sh.K[pb[q]][pa[q]] = kacc[q];
+    }
   for (int idx = tid; idx < c * c; idx += eig::NT) sh.Q[idx / c][idx % c] = (idx / c == idx % c) ? 1.0 : 0.0;
   __syncthreads();
   if (wave == 0) {  // cyclic Jacobi, lane r owns row r of K and Q
@@ -1408,7 +1441,8 @@ int launch_heev_lowest(int n, const void* a, int nev, double* w, void* zt, void*
   for (int round = 0; round < eig::kInverseRounds; ++round) {
     hipLaunchKernelGGL(k_tri_inverse, dim3(m), dim3(64), 8 * (size_t)n * sizeof(double), s, d, e, n, m, round,
                        wb, scal, S, lu);
-    hipLaunchKernelGGL(k_tri_cluster, dim3(nev), dim3(eig::NT), 0, s, d, e, n, m, nev, round,
+    hipLaunchKernelGGL(k_tri_cluster, dim3(nev), dim3(eig::NT), 2 * eig::kClusterMax * (kRRCols + 1) * sizeof(double),
+                       s, d, e, n, m, nev, round,
                        round + 1 == eig::kInverseRounds ? 1 : 0, wb, scal, w, S, lu);
   }
   if (hipGetLastError() != hipSuccess) {
