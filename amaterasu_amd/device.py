@@ -10,12 +10,15 @@ objects the caller already has (amaterasu's ``PlaneWaveSystem.wave_spheres`` and
 from __future__ import annotations
 
 import ctypes
+import itertools
 
 import numpy as np
 import torch
 
 from . import autotune, ops
 from ._lib import FP32, FP64, AmtError, check, lib
+
+_SERIAL = itertools.count(1)
 
 
 def _blocks_from_labels(labels, nproj: int):
@@ -82,6 +85,10 @@ class DevicePlan:
         self._mat64 = None
         self._proj_src = projectors
         self.nproj = 0 if projectors is None else len(projectors[0].labels)
+        self._shared_veff = None
+        #: unique over the process; renewed whenever a device table a captured graph could
+        #: hold (projector tables, shared potential buffers) is replaced
+        self.serial = next(_SERIAL)
 
     def __del__(self):
         h = getattr(self, "_handle", None)
@@ -110,6 +117,7 @@ class DevicePlan:
         self._kb = {}
         self._mat64 = None
         self.nproj = 0 if projectors is None else len(projectors[0].labels)
+        self.serial = next(_SERIAL)
 
     def _attach(self, prec: int):
         """Hand this precision's projector tables to the library plan (amt_plan_set_projectors);
@@ -160,6 +168,20 @@ class DevicePlan:
         if coupling is None:
             self._kb[prec] = tables
         return tables
+
+    def shared_potential(self, pot: torch.Tensor) -> dict:
+        """Copy a (nspin, *grid) potential into this plan's persistent FP64 and FP32 device
+        buffers (allocated on first use, reallocated only when nspin changes) and return
+        them as {FP64: ..., FP32: ...}. Graphs captured over H.psi keep reading them."""
+        buf = self._shared_veff
+        if buf is None or tuple(buf[FP64].shape) != tuple(pot.shape):
+            buf = {FP64: torch.empty(tuple(pot.shape), dtype=torch.float64, device=self.device),
+                   FP32: torch.empty(tuple(pot.shape), dtype=torch.float32, device=self.device)}
+            self._shared_veff = buf
+            self.serial = next(_SERIAL)
+        buf[FP64].copy_(pot)
+        buf[FP32].copy_(buf[FP64])
+        return buf
 
     # ---------------------------------------------------------------- workspaces
     def sticks(self, prec: int, nrows: int) -> torch.Tensor:

@@ -23,6 +23,8 @@ numpy in gives numpy out.
 
 from __future__ import annotations
 
+import os
+from collections import OrderedDict
 from dataclasses import dataclass
 
 import numpy as np
@@ -236,8 +238,150 @@ class _Operator:
         return self.fn(block).to(self.dtype).contiguous()
 
 
+def graphs_enabled() -> bool:
+    """Whether the SCF replays Davidson passes from captured graphs (``AMT_DAVIDSON_GRAPHS=1``)."""
+    return os.environ.get("AMT_DAVIDSON_GRAPHS", "0") == "1"
+
+
+def graphs_per_operator() -> int:
+    """Graphs kept per operator identity (``AMT_DAVIDSON_GRAPHS_MAX``, default 512); the least
+    recently replayed one goes first. A cached graph holds host-side nodes only."""
+    return max(int(os.environ.get("AMT_DAVIDSON_GRAPHS_MAX", "512")), 1)
+
+#: Operator identities whose buffers and graphs are kept at once.
+GRAPH_OPERATORS = 8
+
+
+class StepGraphs:
+    """The expansion half of Davidson passes captured as HIP graphs, for one operator.
+
+    One graph per (m, nn) -- m basis rows before the pass, nn new directions -- holds every
+    launch between the two host reads of a pass (the active set, the orthogonalisation's
+    Cholesky status): H on the nn new rows, the new columns of the projected matrix, the
+    Rayleigh-Ritz eigensolve and the two rotations, and the residuals of the new Ritz pairs.
+    The basis, H.basis and projected-matrix buffers belong to this object, so their
+    addresses, baked into every graph, never change; the operator's own device inputs
+    (potential, projector tables) must stay in place while its identity is unchanged.
+    A shape is run eagerly the first time it is met (that run sizes every cached workspace
+    the graph will use) and captured the second time; a capture during which a cached
+    workspace had to grow is dropped, because the graph would hold the old buffer's
+    address. Replays and eager runs issue the same kernels in the same order, so both give
+    the same numbers. Graphs hold references to the workspaces they were captured with, so
+    those buffers outlive any later growth of the cache.
+
+    Every graph ends by copying its Ritz values, vectors, H.vectors, residuals and residual
+    norms into this object's output buffers, so the graphs of one object share one memory
+    pool of temporaries and a cached graph holds no device memory of its own. A pass
+    consumes those outputs (preconditioner, collapse, the returned copies) before the next
+    replay of any graph.
+    """
+
+    def __init__(self, cap: int, npw: int, nbands: int, dtype):
+        self.basis = torch.zeros((cap, npw), dtype=dtype, device="cuda")
+        self.h_basis = torch.zeros_like(self.basis)
+        self.proj = torch.zeros((cap, cap), dtype=torch.complex128, device="cuda")
+        blk = torch.empty((nbands, npw), dtype=dtype, device="cuda")
+        real = torch.empty(nbands, dtype=torch.float64, device="cuda")
+        self.out = (real, blk, torch.empty_like(blk), torch.empty_like(blk), torch.empty_like(real))
+        self.graphs: OrderedDict = OrderedDict()
+        self.seen: set = set()
+        self.pool = torch.cuda.graph_pool_handle()
+        self.captures = 0
+        self.replays = 0
+
+    def _capture(self, body):
+        graph = torch.cuda.CUDAGraph()
+        main = torch.cuda.current_stream()
+        side = _capture_stream()
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            graph.capture_begin(pool=self.pool)
+            try:
+                for dst, src in zip(self.out, body()):
+                    dst.copy_(src)
+            finally:
+                graph.capture_end()
+        main.wait_stream(side)
+        return graph, self.out
+
+    def run(self, key, body):
+        entry = self.graphs.get(key)
+        if entry is not None:
+            self.graphs.move_to_end(key)
+            entry[0].replay()
+            self.replays += 1
+            return entry[1]
+        if key not in self.seen:
+            self.seen.add(key)
+            return body()
+        before = dict(ops._WORKSPACES)
+        graph, out = self._capture(body)
+        after = ops._WORKSPACES
+        if len(after) != len(before) or any(after.get(k) is not v for k, v in before.items()):
+            del graph
+            return body()
+        self.graphs[key] = (graph, out, list(after.values()))
+        if len(self.graphs) > graphs_per_operator():
+            self.graphs.popitem(last=False)
+        self.captures += 1
+        graph.replay()
+        self.replays += 1
+        return out
+
+
+_STEP_GRAPHS: OrderedDict = OrderedDict()
+_CAPTURE_STREAMS: dict = {}
+
+
+def _capture_stream() -> torch.cuda.Stream:
+    """A side stream per device (default priority) for captures: the null stream cannot capture."""
+    dev = torch.cuda.current_device()
+    st = _CAPTURE_STREAMS.get(dev)
+    if st is None:
+        st = _CAPTURE_STREAMS[dev] = torch.cuda.Stream()
+    return st
+
+
+def step_graphs(graph_key, cap: int, npw: int, nbands: int, dtype) -> StepGraphs:
+    """The (cached) :class:`StepGraphs` of one operator identity and block shape."""
+    key = (graph_key, cap, npw, nbands, dtype, torch.cuda.current_device())
+    sg = _STEP_GRAPHS.get(key)
+    if sg is None:
+        sg = StepGraphs(cap, npw, nbands, dtype)
+        _STEP_GRAPHS[key] = sg
+        if len(_STEP_GRAPHS) > GRAPH_OPERATORS:
+            _STEP_GRAPHS.popitem(last=False)
+    else:
+        _STEP_GRAPHS.move_to_end(key)
+    return sg
+
+
+def graph_stats() -> dict:
+    """Captures and replays of every cached :class:`StepGraphs` (for tests and traces)."""
+    return {"captures": sum(g.captures for g in _STEP_GRAPHS.values()),
+            "replays": sum(g.replays for g in _STEP_GRAPHS.values()),
+            "graphs": sum(len(g.graphs) for g in _STEP_GRAPHS.values())}
+
+
+def clear_graphs() -> None:
+    _STEP_GRAPHS.clear()
+
+
+def _expand(op, basis, h_basis, proj, m: int, nn: int, nbands: int):
+    """H on the new rows m..m+nn, the projected matrix's new columns, Rayleigh-Ritz and the
+    residuals of the new Ritz pairs (``davidson.py:80-82,99-101``)."""
+    h_basis[m : m + nn] = op(basis[m : m + nn])
+    cols = _gram(basis[: m + nn], h_basis[m : m + nn]).conj()  # proj[:, new]
+    proj[:m, m : m + nn] = cols[:m]
+    proj[m : m + nn, :m] = cols[:m].conj().T
+    proj[m : m + nn, m : m + nn] = _hermitian(cols[m:])
+    eps, vec, hvec = _ritz(proj[: m + nn, : m + nn], basis[: m + nn], h_basis[: m + nn], nbands)
+    res, norms = ops.residual(hvec, vec, eps)
+    return eps, vec, hvec, res, norms
+
+
 def davidson(apply_hamiltonian, guess, kinetic, *, tolerance: float = 1e-8, max_iterations: int = 40,
-             subspace_factor: int = 4, precision="fp64") -> DavidsonResult:
+             subspace_factor: int = 4, precision="fp64", graph_key=None) -> DavidsonResult:
     """Lowest eigenpairs of a Hermitian operator from a starting block (``davidson.py:48-111``).
 
     The basis and H.basis live in preallocated ``[max_subspace][npw]`` device buffers, and
@@ -245,6 +389,11 @@ def davidson(apply_hamiltonian, guess, kinetic, *, tolerance: float = 1e-8, max_
     only the columns of the new directions (8 m n_new npw flops instead of 8 m^2 npw) --
     the same matrix the reference rebuilds from scratch (``subspace.py:72``), up to
     rounding. After a collapse the block of the Ritz vectors is recomputed.
+
+    ``graph_key`` (any hashable) asks for the expansion half of each pass to be replayed
+    from HIP graphs (:class:`StepGraphs`): the caller promises that every device input of
+    ``apply_hamiltonian`` stays at the same address with the same contents while it passes
+    the same key. The iteration, and every number it produces, is the eager one's.
     """
     host = not isinstance(guess, torch.Tensor)
     cdt = torch.complex128 if precision in ("fp64", FP64, None) else torch.complex64
@@ -255,19 +404,28 @@ def davidson(apply_hamiltonian, guess, kinetic, *, tolerance: float = 1e-8, max_
     nbands, npw = v0.shape
     max_subspace = min(max(subspace_factor, 2) * nbands, npw)
     cap = max(max_subspace, 2 * nbands)
-    basis = torch.zeros((cap, npw), dtype=cdt, device="cuda")
-    h_basis = torch.zeros_like(basis)
-    proj = torch.zeros((cap, cap), dtype=torch.complex128, device="cuda")
+    sg = None
+    if graph_key is not None:
+        sg = step_graphs(graph_key, cap, npw, nbands, cdt)
+        basis, h_basis, proj = sg.basis, sg.h_basis, sg.proj
+    else:
+        basis = torch.zeros((cap, npw), dtype=cdt, device="cuda")
+        h_basis = torch.zeros_like(basis)
+        proj = torch.zeros((cap, cap), dtype=torch.complex128, device="cuda")
     basis[:nbands] = v0
     h_basis[:nbands] = op(v0)
+    if sg is not None and op.host:
+        raise AmtError("davidson: graph replay needs an operator that takes device blocks")
     m = nbands
     proj[:m, :m] = _hermitian(_gram(basis[:m], h_basis[:m]).conj())
     eps, vec, hvec = _ritz(proj[:m, :m], basis[:m], h_basis[:m], nbands)
+    res, next_norms = ops.residual(hvec, vec, eps)
     norms = torch.full((nbands,), float("inf"), dtype=torch.float64, device="cuda")
     iteration = 0
     for step in range(1, max_iterations + 1):
         iteration = step
-        res, norms = ops.residual(hvec, vec, eps)
+        # the residuals of this pass's Ritz pairs (a graph's replay overwrites its outputs)
+        norms = next_norms if sg is None else next_norms.clone()
         active = torch.nonzero(norms > tolerance).flatten()
         if active.numel() == 0:
             break
@@ -278,18 +436,19 @@ def davidson(apply_hamiltonian, guess, kinetic, *, tolerance: float = 1e-8, max_
             break
         nn = min(nn, cap - m)
         basis[m : m + nn] = new[:nn]
-        h_basis[m : m + nn] = op(new[:nn].contiguous())
-        cols = _gram(basis[: m + nn], h_basis[m : m + nn]).conj()  # proj[:, new]
-        proj[:m, m : m + nn] = cols[:m]
-        proj[m : m + nn, :m] = cols[:m].conj().T
-        proj[m : m + nn, m : m + nn] = _hermitian(cols[m:])
+        body = (lambda m=m, nn=nn: _expand(op, basis, h_basis, proj, m, nn, nbands))
+        if sg is not None and HEEV_MIN <= m + nn <= HEEV_MAX:
+            eps, vec, hvec, res, next_norms = sg.run((m, nn), body)
+        else:
+            eps, vec, hvec, res, next_norms = body()
         m += nn
-        eps, vec, hvec = _ritz(proj[:m, :m], basis[:m], h_basis[:m], nbands)
         if m + nbands > max_subspace:
             basis[:nbands] = vec
             h_basis[:nbands] = hvec
             m = nbands
             proj[:m, :m] = _hermitian(_gram(basis[:m], h_basis[:m]).conj())
+    if sg is not None:  # graph outputs are overwritten by the next replay
+        eps, vec = eps.clone(), vec.clone()
     converged = bool((norms <= tolerance).all().item())
     if host:
         return DavidsonResult(eps.cpu().numpy(), vec.to(torch.complex128).cpu().numpy(), norms.cpu().numpy(),
@@ -330,5 +489,6 @@ def solve(name, apply_hamiltonian, guess, kinetic, *, apply_overlap=None, **kwar
     return davidson(apply_hamiltonian, guess, kinetic, **kwargs)
 
 
-__all__ = ["DavidsonResult", "available_eigensolvers", "davidson", "orthogonalize_against", "orthonormalize",
-           "random_bands", "rayleigh_ritz", "solve", "teter_payne_allan"]
+__all__ = ["DavidsonResult", "StepGraphs", "available_eigensolvers", "clear_graphs", "davidson", "graph_stats",
+           "graphs_enabled", "orthogonalize_against", "orthonormalize", "random_bands", "rayleigh_ritz", "solve",
+           "step_graphs", "teter_payne_allan"]

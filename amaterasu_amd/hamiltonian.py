@@ -58,7 +58,12 @@ def _precision(name) -> int:
 class Hamiltonian:
     """A Kohn-Sham Hamiltonian at one fixed effective potential, applied on the GPU."""
 
-    def __init__(self, system, effective_potential, coupling=None, *, precision="fp64"):
+    def __init__(self, system, effective_potential, coupling=None, *, precision="fp64",
+                 shared_potential: bool = False):
+        """``shared_potential=True`` copies the potential into the plan's own device buffers
+        (:meth:`DevicePlan.shared_potential`) instead of fresh tensors: every Hamiltonian of
+        the plan built that way then reads the potential given last, at fixed addresses, and
+        ``graph_key`` names that state for Davidson graph replay (None otherwise)."""
         if isinstance(effective_potential, torch.Tensor):
             pot = effective_potential.to(torch.float64)
         else:
@@ -72,8 +77,14 @@ class Hamiltonian:
         self.coupling = coupling
         self.precision = _precision(precision)
         self.plan = device_plan(system)
-        self._veff = {FP64: pot.to("cuda").contiguous()}
-        self._veff[FP32] = self._veff[FP64].to(torch.float32)
+        self.graph_key = None
+        if shared_potential:
+            self._veff = self.plan.shared_potential(pot)
+            if coupling is None:
+                self.graph_key = ("shared-potential", self.plan.serial)
+        else:
+            self._veff = {FP64: pot.to("cuda").contiguous()}
+            self._veff[FP32] = self._veff[FP64].to(torch.float32)
 
     @property
     def effective_potential(self) -> np.ndarray:

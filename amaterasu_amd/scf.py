@@ -29,7 +29,7 @@ import torch
 from . import ops
 from .batched import solve_batched
 from .density import _trivial_symmetry
-from .eigen import davidson, orthonormalize
+from .eigen import davidson, graphs_enabled, orthonormalize
 from .hamiltonian import Hamiltonian, device_plan
 from .xc import GridOps, XCFunctional
 
@@ -62,6 +62,10 @@ class SCFSettings:
     #: Run the Davidson of all k-points of a spin channel together (one launch per kernel
     #: over every k); each k-point still follows its own iteration.
     batched_k: bool = True
+    #: Replay the expansion half of each per-k Davidson pass from HIP graphs captured per
+    #: (subspace size, new directions) (eigen.StepGraphs); the potential then lives in one
+    #: device buffer per plan, rewritten every iteration. Default: ``AMT_DAVIDSON_GRAPHS=1``.
+    davidson_graphs: bool = field(default_factory=graphs_enabled)
 
 
 @dataclass
@@ -472,7 +476,7 @@ def run_scf(system, settings: SCFSettings, *, density=None, wavefunctions=None, 
             cdt = torch.complex128
             wf = wf.to(cdt)
         prec = "fp64" if cdt == torch.complex128 else "fp32"
-        ham = Hamiltonian(system, pot, precision=prec)
+        ham = Hamiltonian(system, pot, precision=prec, shared_potential=settings.davidson_graphs)
         tol = float(np.clip(1e-3 * min(resid, 1.0), 1e-11, 1e-6))
         if prec == "fp32":
             tol = max(tol, settings.fp32_tol_floor)
@@ -489,8 +493,9 @@ def run_scf(system, settings: SCFSettings, *, density=None, wavefunctions=None, 
                 continue
             for ik in range(nk):
                 n = plan.npw[ik]
+                gkey = (ham.graph_key, ik, sp) if ham.graph_key is not None else None
                 r = davidson(lambda b, ik=ik, sp=sp: ham.apply(b, ik, sp), wf[sp, ik, :, :n].contiguous(),
-                             kin[ik, :n], tolerance=tol, max_iterations=steps, precision=prec)
+                             kin[ik, :n], tolerance=tol, max_iterations=steps, precision=prec, graph_key=gkey)
                 eig[sp, ik] = r.eigenvalues
                 wf[sp, ik, :, :n] = r.coefficients.to(cdt)
                 dav_steps += r.iterations

@@ -144,6 +144,7 @@ def main(argv=None) -> int:
     ap.add_argument("--max-steps", type=int, default=60)
     ap.add_argument("--fp32-tol-floor", type=float, default=None, help="eigensolver tolerance floor of FP32 blocks")
     ap.add_argument("--fp64-below", type=float, default=None, help="mixed: density residual of the switch to FP64")
+    ap.add_argument("--runs", type=int, default=1, help="SCFs one after another in this process (each timed)")
     a = ap.parse_args(argv)
     import torch
 
@@ -167,12 +168,22 @@ def main(argv=None) -> int:
     t = t_setup
     print(f"system ready: {len(system.symbols)} atoms, grid {system.grid.shape}, "
           f"npw {[len(sp.g2) for sp in system.wave_spheres]}, {system.nbands} bands", flush=True)
-    r = run_scf(system, settings, density=rho, log=lambda line: print(f"{time.perf_counter() - t:8.3f}s {line}",
-                                                                      flush=True))
-    torch.cuda.synchronize()
-    print(f"converged {r.converged} after {r.iterations} iterations, E {r.energies.total:.10f} Ha, "
-          f"{time.perf_counter() - t:.3f} s", flush=True)
-    return 0 if r.converged else 1
+    ok = True
+    for run in range(max(a.runs, 1)):
+        if run:
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+        r = run_scf(system, settings, density=rho,
+                    log=lambda line: print(f"{time.perf_counter() - t:8.3f}s {line}", flush=True))
+        torch.cuda.synchronize()
+        print(f"converged {r.converged} after {r.iterations} iterations, E {r.energies.total:.10f} Ha, "
+              f"{time.perf_counter() - t:.3f} s", flush=True)
+        if settings.davidson_graphs:
+            from .eigen import graph_stats
+
+            print(f"Davidson graphs: {graph_stats()}", flush=True)
+        ok = ok and r.converged
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":
