@@ -472,6 +472,122 @@ __global__ __launch_bounds__(64) void k_hetrd_step_w64(double2* __restrict__ A, 
   }
 }
 
+// The last kTailLen steps of the reduction in one workgroup of four wavefronts: the trailing
+// matrix (at most 64 x 64) stays in registers, lane r of every wavefront holding row r and
+// wavefront q columns 16 q .. 16 q + 15, and each step takes two workgroup barriers (the
+// new column, the partial products) instead of a launch. Every wavefront keeps the whole of
+// v, w and p in its lanes (lane r holds element r) and reduces them itself, in the same
+// order, so all four hold identical values; a wavefront broadcasts v and w to its own
+// columns through a private LDS copy. Takes over at step j0 = n - 1 - L0 with the state a
+// step launch would find (rows j0+1.. updated through step j0-1, v_j0 in V, tau_j0, p_j0)
+// and writes d, e, V and taus of steps j0 .. n-2 as the step launches would.
+constexpr int kTailLen = 64;
+
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_wave_barrier();
+  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int K>
+__device__ __forceinline__ double2 reg_element(const double2 (&x)[K], int k) {
+  double vx = x[0].x, vy = x[0].y;
+#pragma unroll
+  for (int u = 1; u < K; ++u) {
+    vx = k == u ? x[u].x : vx;
+    vy = k == u ? x[u].y : vy;
+  }
+  return make_double2(vx, vy);
+}
+
+__global__ __launch_bounds__(256) void k_hetrd_tail(double2* __restrict__ A, int n, int j0, double2* __restrict__ V,
+                                                    double2* __restrict__ taus, double* __restrict__ d,
+                                                    double* __restrict__ e, const double2* __restrict__ p_in) {
+  __shared__ double2 vw[4][2][64];  // per-wavefront copies of v and w
+  __shared__ double2 col[64];       // the column below the diagonal of the next step
+  __shared__ double2 part[4][64];   // per-wavefront partial products M v
+  const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
+  const int L0 = n - j0 - 1;  // trailing length at step j0, <= kTailLen (checked by the host)
+  const int c0 = 16 * wq;
+  const double2 zero = make_double2(0.0, 0.0);
+  double2 m[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int c = c0 + k;
+    double2 a = zero;
+    if (lane < L0 && c < L0) a = A[(long long)(j0 + 1 + lane) * n + j0 + 1 + c];
+    m[k] = make_double2(a.x, a.y);
+  }
+  double2 v = zero, p = zero;
+  if (lane < L0) {
+    v = V[(long long)j0 * n + lane];
+    p = p_in[lane];
+  }
+  double2 tau = taus[j0];
+  for (int s = 0; s < L0; ++s) {
+    const int j = j0 + s;
+    // step j: v, p hold elements r >= s (trailing index r - s), zero below
+    const double2 K = zmul(make_double2(-0.5 * tau.x, -0.5 * tau.y), wave_sum2(zmul(zconj(p), v)));
+    const double2 w = cadd(p, zmul(K, v));
+    vw[wq][0][lane] = v;
+    vw[wq][1][lane] = w;
+    wave_lds_order();
+    if (s < c0 + 16) {  // columns below s see v = w = 0 and stay as they are
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const double2 vc = vw[wq][0][c0 + k], wc = vw[wq][1][c0 + k];
+        m[k] = csub(m[k], cadd(zmul(v, zconj(wc)), zmul(w, zconj(vc))));
+      }
+    }
+    const bool owner = wq == (s >> 4);
+    if (owner && lane == s) d[j + 1] = reg_element(m, s & 15).x;
+    if (s == L0 - 1) break;
+    // zlarfg on the updated column s below the diagonal (rows s+1 ..)
+    if (owner) col[lane] = reg_element(m, s & 15);
+    __syncthreads();
+    double2 x = zero;
+    if (lane > s) x = col[lane];
+    const double2 alpha = col[s + 1];
+    const double xnorm2 = wave_sum2(make_double2(lane > s + 1 ? x.x * x.x + x.y * x.y : 0.0, 0.0)).x;
+    double beta;
+    double2 tau2, sc;
+    if (xnorm2 == 0.0 && alpha.y == 0.0) {
+      beta = alpha.x;
+      tau2 = zero;
+      sc = zero;
+    } else {
+      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
+      beta = alpha.x >= 0.0 ? -nrm : nrm;
+      tau2 = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      const double2 dd = make_double2(alpha.x - beta, alpha.y);
+      const double den = dd.x * dd.x + dd.y * dd.y;
+      sc = make_double2(dd.x / den, -dd.y / den);
+    }
+    double2 vn = zmul(x, sc);  // x is zero up to row s
+    if (lane == s + 1) vn = make_double2(1.0, 0.0);
+    if (wq == 0 && lane > s && lane < L0) V[(long long)(j + 1) * n + lane - s - 1] = vn;
+    if (threadIdx.x == 0) {
+      taus[j + 1] = tau2;
+      e[j + 1] = beta;
+    }
+    // p_{j+1} = tau2 M[s+1:, s+1:] v_{j+1}: this wavefront's columns, then the four partials
+    vw[wq][0][lane] = vn;
+    wave_lds_order();
+    double2 acc = zero;
+    if (s + 1 < c0 + 16) {  // v_{j+1} is zero at columns up to s
+#pragma unroll
+      for (int k = 0; k < 16; ++k) acc = cadd(acc, zmul(m[k], vw[wq][0][c0 + k]));
+    }
+    part[wq][lane] = acc;
+    __syncthreads();
+    const double2 tot = cadd(cadd(part[0][lane], part[1][lane]), cadd(part[2][lane], part[3][lane]));
+    p = zmul(tau2, tot);
+    if (lane <= s) p = zero;
+    v = vn;
+    tau = tau2;
+  }
+}
+
 // The step for 1024 < n <= 2048: 32 row elements per lane take 128 VGPRs, so v and w live
 // in LDS only (not also in registers) and the new column reuses v's LDS after one more
 // barrier: at most 256 registers and 2n elements of LDS per workgroup (56 KB at n = 1760),
@@ -1378,6 +1494,10 @@ static void enqueue_tridiag(hipStream_t s, int n, double2* A, double2* V, double
     const double2* pin = (j & 1) ? p1 : p0;
     double2* pout = (j & 1) ? p0 : p1;
     const int len = n - j - 1, rows = n - j - 2;
+    if (len <= kTailLen) {  // steps j .. n-2 in one launch
+      hipLaunchKernelGGL(k_hetrd_tail, dim3(1), dim3(256), 0, s, A, n, j, V, taus, d, e, pin);
+      break;
+    }
     const dim3 grid(rows > 0 ? (rows + 3) / 4 : 1);
     if (len > eig::MAXN / 2)
       hipLaunchKernelGGL(k_hetrd_step_wide, grid, dim3(eig::NT), lds2, s, A, n, j, V, taus, d, e, pin, pout);

@@ -21,8 +21,8 @@ def _hermitian(n, seed, degenerate=False):
     return (x + x.conj().T) / 2
 
 
-@pytest.mark.parametrize("n,nev", [(2, 1), (3, 3), (17, 5), (160, 40), (333, 100), (640, 160), (1024, 256),
-                                   (1025, 300), (1760, 440), (2048, 512)])
+@pytest.mark.parametrize("n,nev", [(2, 1), (3, 3), (17, 5), (64, 20), (65, 65), (66, 30), (160, 40), (333, 100),
+                                   (640, 160), (1024, 256), (1025, 300), (1760, 440), (2048, 512)])
 def test_heev_matches_lapack(n, nev):
     from amaterasu_amd import ops
 
@@ -34,6 +34,26 @@ def test_heev_matches_lapack(n, nev):
     assert np.abs(w - ref).max() < 1e-13 * scale * max(1, n / 100)
     assert np.abs(a @ z - z * w).max() < 1e-12 * scale * max(1, n / 100)
     assert np.abs(z.conj().T @ z - np.eye(nev)).max() < 1e-12 * max(1, n / 100)
+
+
+@pytest.mark.parametrize("n", [40, 130])
+def test_heev_zero_columns_in_the_last_steps(n):
+    """A Hermitian block over a diagonal block: the last columns of the reduction (the last 64
+    run in one workgroup) have nothing below the diagonal, so their reflectors are H = I
+    (tau = 0), and the eigenvalues of the diagonal block are exact."""
+    from amaterasu_amd import ops
+
+    rng = np.random.default_rng(n)
+    a = np.zeros((n, n), dtype=complex)
+    h = n - 30
+    a[:h, :h] = _hermitian(h, n + 1)
+    a[h:, h:] = np.diag(rng.uniform(-3.0, 3.0, 30))
+    nev = n // 2
+    w, zt = ops.heev_lowest(torch.from_numpy(a).cuda(), nev)
+    w, z = w.cpu().numpy(), zt.cpu().numpy().T
+    assert np.abs(w - np.linalg.eigvalsh(a)[:nev]).max() < 1e-13 * 10
+    assert np.abs(a @ z - z * w).max() < 1e-12 * 10
+    assert np.abs(z.conj().T @ z - np.eye(nev)).max() < 1e-12
 
 
 def test_heev_degenerate_clusters():
