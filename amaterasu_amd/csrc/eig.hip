@@ -472,21 +472,49 @@ __global__ __launch_bounds__(64) void k_hetrd_step_w64(double2* __restrict__ A, 
   }
 }
 
-// The last kTailLen steps of the reduction in one workgroup of four wavefronts: the trailing
+// The last kTailLen steps of the reduction in one workgroup of NW wavefronts: the trailing
 // matrix (at most 64 x 64) stays in registers, lane r of every wavefront holding row r and
-// wavefront q columns 16 q .. 16 q + 15, and each step takes two workgroup barriers (the
-// new column, the partial products) instead of a launch. Every wavefront keeps the whole of
-// v, w and p in its lanes (lane r holds element r) and reduces them itself, in the same
-// order, so all four hold identical values; a wavefront broadcasts v and w to its own
-// columns through a private LDS copy. Takes over at step j0 = n - 1 - L0 with the state a
-// step launch would find (rows j0+1.. updated through step j0-1, v_j0 in V, tau_j0, p_j0)
-// and writes d, e, V and taus of steps j0 .. n-2 as the step launches would.
+// wavefront q columns 64/NW q .. 64/NW (q + 1) - 1, and each step takes one workgroup barrier instead of
+// a launch. Every wavefront keeps the whole of v, w and p in its lanes (lane r holds element
+// r) and reduces them itself, in the same order, so all of them hold identical values; a
+// wavefront broadcasts v and w to its own columns through a private LDS copy. The pivot
+// column of the next step is published by the wavefront that holds it before the barrier
+// that closes a step, and every wavefront applies the next step's update to it itself (the
+// same arithmetic as the owner's), so the reflector needs no barrier of its own. Takes over
+// at step j0 = n - 1 - L0 with the state a step launch would find (rows j0+1.. updated
+// through step j0-1, v_j0 in V, tau_j0, p_j0) and writes d, e, V and taus of steps j0 .. n-2
+// as the step launches would.
 constexpr int kTailLen = 64;
+constexpr int kTailWaves = 8;
 
 __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_wave_barrier();
   __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
   __builtin_amdgcn_wave_barrier();
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_f64(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Sum over the wavefront, the same bits in every lane: rotations within each row of 16 lanes
+// (commutative pairs, so every lane of a row ends with the same sum), then the four row sums
+// added in a fixed order.
+__device__ __forceinline__ double wave_allsum(double x) {
+  x += dpp_f64<0x128>(x);  // row_ror:8
+  x += dpp_f64<0x124>(x);  // row_ror:4
+  x += dpp_f64<0x122>(x);  // row_ror:2
+  x += dpp_f64<0x121>(x);  // row_ror:1
+  return (lane_f64(x, 0) + lane_f64(x, 16)) + (lane_f64(x, 32) + lane_f64(x, 48));
 }
 
 template <int K>
@@ -500,55 +528,86 @@ __device__ __forceinline__ double2 reg_element(const double2 (&x)[K], int k) {
   return make_double2(vx, vy);
 }
 
-__global__ __launch_bounds__(256) void k_hetrd_tail(double2* __restrict__ A, int n, int j0, double2* __restrict__ V,
-                                                    double2* __restrict__ taus, double* __restrict__ d,
-                                                    double* __restrict__ e, const double2* __restrict__ p_in) {
-  __shared__ double2 vw[4][2][64];  // per-wavefront copies of v and w
-  __shared__ double2 col[64];       // the column below the diagonal of the next step
-  __shared__ double2 part[4][64];   // per-wavefront partial products M v
+template <int NW, int RPL>
+__global__ __launch_bounds__(64 * NW) void k_hetrd_tail(double2* __restrict__ A, int n, int j0,
+                                                        double2* __restrict__ V, double2* __restrict__ taus,
+                                                        double* __restrict__ d, double* __restrict__ e,
+                                                        const double2* __restrict__ p_in) {
+  constexpr int TL = 64 * RPL;  // rows (and columns) held
+  constexpr int CW = TL / NW;   // columns per wavefront
+  __shared__ double2 vw[NW][2][TL];    // per-wavefront copies of v and w
+  __shared__ double2 col[2][TL];       // the next pivot column before its update, by step parity
+  __shared__ double2 part[2][NW][TL];  // per-wavefront partial products M v, by step parity
   const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
-  const int L0 = n - j0 - 1;  // trailing length at step j0, <= kTailLen (checked by the host)
-  const int c0 = 16 * wq;
+  const int L0 = n - j0 - 1;  // trailing length at step j0, <= TL (checked by the host)
+  const int c0 = CW * wq;
   const double2 zero = make_double2(0.0, 0.0);
-  double2 m[16];
+  double2 m[RPL][CW];  // rows lane + 64 q, columns c0 ..
+  double2 v[RPL], p[RPL];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const int c = c0 + k;
-    double2 a = zero;
-    if (lane < L0 && c < L0) a = A[(long long)(j0 + 1 + lane) * n + j0 + 1 + c];
-    m[k] = make_double2(a.x, a.y);
-  }
-  double2 v = zero, p = zero;
-  if (lane < L0) {
-    v = V[(long long)j0 * n + lane];
-    p = p_in[lane];
+  for (int q = 0; q < RPL; ++q) {
+    const int r = lane + 64 * q;
+#pragma unroll
+    for (int k = 0; k < CW; ++k) {
+      const int c = c0 + k;
+      double2 a = zero;
+      if (r < L0 && c < L0) a = A[(long long)(j0 + 1 + r) * n + j0 + 1 + c];
+      m[q][k] = make_double2(a.x, a.y);
+    }
+    v[q] = zero;
+    p[q] = zero;
+    if (r < L0) {
+      v[q] = V[(long long)j0 * n + r];
+      p[q] = p_in[r];
+    }
   }
   double2 tau = taus[j0];
+  if (wq == 0) {
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) col[0][lane + 64 * q] = m[q][0];
+  }
+  __syncthreads();
   for (int s = 0; s < L0; ++s) {
     const int j = j0 + s;
     // step j: v, p hold elements r >= s (trailing index r - s), zero below
-    const double2 K = zmul(make_double2(-0.5 * tau.x, -0.5 * tau.y), wave_sum2(zmul(zconj(p), v)));
-    const double2 w = cadd(p, zmul(K, v));
-    vw[wq][0][lane] = v;
-    vw[wq][1][lane] = w;
-    wave_lds_order();
-    if (s < c0 + 16) {  // columns below s see v = w = 0 and stay as they are
+    double2 pv = zero;
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
+    for (int q = 0; q < RPL; ++q) pv = cadd(pv, zmul(zconj(p[q]), v[q]));
+    const double2 K =
+        zmul(make_double2(-0.5 * tau.x, -0.5 * tau.y), make_double2(wave_allsum(pv.x), wave_allsum(pv.y)));
+    double2 w[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      w[q] = cadd(p[q], zmul(K, v[q]));
+      vw[wq][0][lane + 64 * q] = v[q];
+      vw[wq][1][lane + 64 * q] = w[q];
+    }
+    wave_lds_order();
+    if (s < c0 + CW) {  // columns below s see v = w = 0 and stay as they are
+#pragma unroll
+      for (int k = 0; k < CW; ++k) {
         const double2 vc = vw[wq][0][c0 + k], wc = vw[wq][1][c0 + k];
-        m[k] = csub(m[k], cadd(zmul(v, zconj(wc)), zmul(w, zconj(vc))));
+#pragma unroll
+        for (int q = 0; q < RPL; ++q)
+          m[q][k] = csub(m[q][k], cadd(zmul(v[q], zconj(wc)), zmul(w[q], zconj(vc))));
       }
     }
-    const bool owner = wq == (s >> 4);
-    if (owner && lane == s) d[j + 1] = reg_element(m, s & 15).x;
+    // the pivot column s after this step's update, as its owner computes it
+    const double2 vs = vw[wq][0][s], ws = vw[wq][1][s];
+    double2 xc[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; ++q)
+      xc[q] = csub(col[s & 1][lane + 64 * q], cadd(zmul(v[q], zconj(ws)), zmul(w[q], zconj(vs))));
+    if (wq == 0 && lane == (s & 63)) d[j + 1] = reg_element(xc, s >> 6).x;
     if (s == L0 - 1) break;
-    // zlarfg on the updated column s below the diagonal (rows s+1 ..)
-    if (owner) col[lane] = reg_element(m, s & 15);
-    __syncthreads();
-    double2 x = zero;
-    if (lane > s) x = col[lane];
-    const double2 alpha = col[s + 1];
-    const double xnorm2 = wave_sum2(make_double2(lane > s + 1 ? x.x * x.x + x.y * x.y : 0.0, 0.0)).x;
+    // zlarfg on the column below the diagonal (rows s+1 ..)
+    const double2 xa = reg_element(xc, (s + 1) >> 6);
+    const double2 alpha = make_double2(lane_f64(xa.x, (s + 1) & 63), lane_f64(xa.y, (s + 1) & 63));
+    double nx = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q)
+      if (lane + 64 * q > s + 1) nx += xc[q].x * xc[q].x + xc[q].y * xc[q].y;
+    const double xnorm2 = wave_allsum(nx);
     double beta;
     double2 tau2, sc;
     if (xnorm2 == 0.0 && alpha.y == 0.0) {
@@ -558,32 +617,58 @@ __global__ __launch_bounds__(256) void k_hetrd_tail(double2* __restrict__ A, int
     } else {
       const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
       beta = alpha.x >= 0.0 ? -nrm : nrm;
-      tau2 = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+      const double rb = 1.0 / beta;
+      tau2 = make_double2((beta - alpha.x) * rb, -alpha.y * rb);
       const double2 dd = make_double2(alpha.x - beta, alpha.y);
-      const double den = dd.x * dd.x + dd.y * dd.y;
-      sc = make_double2(dd.x / den, -dd.y / den);
+      const double rden = 1.0 / (dd.x * dd.x + dd.y * dd.y);
+      sc = make_double2(dd.x * rden, -dd.y * rden);
     }
-    double2 vn = zmul(x, sc);  // x is zero up to row s
-    if (lane == s + 1) vn = make_double2(1.0, 0.0);
-    if (wq == 0 && lane > s && lane < L0) V[(long long)(j + 1) * n + lane - s - 1] = vn;
+    double2 vn[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int r = lane + 64 * q;
+      vn[q] = zero;
+      if (r > s + 1) vn[q] = zmul(xc[q], sc);
+      if (r == s + 1) vn[q] = make_double2(1.0, 0.0);
+      if (wq == 0 && r > s && r < L0) V[(long long)(j + 1) * n + r - s - 1] = vn[q];
+    }
     if (threadIdx.x == 0) {
       taus[j + 1] = tau2;
       e[j + 1] = beta;
     }
-    // p_{j+1} = tau2 M[s+1:, s+1:] v_{j+1}: this wavefront's columns, then the four partials
-    vw[wq][0][lane] = vn;
-    wave_lds_order();
-    double2 acc = zero;
-    if (s + 1 < c0 + 16) {  // v_{j+1} is zero at columns up to s
+    // publish the next pivot column (after this step's update) for the next step
+    if (wq == (s + 1) / CW) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) acc = cadd(acc, zmul(m[k], vw[wq][0][c0 + k]));
+      for (int q = 0; q < RPL; ++q) col[(s + 1) & 1][lane + 64 * q] = reg_element(m[q], (s + 1) % CW);
     }
-    part[wq][lane] = acc;
+    // p_{j+1} = tau2 M[s+1:, s+1:] v_{j+1}: this wavefront's columns, then the partials
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) vw[wq][0][lane + 64 * q] = vn[q];
+    wave_lds_order();
+    double2 acc[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) acc[q] = zero;
+    if (s + 1 < c0 + CW) {  // v_{j+1} is zero at columns up to s
+#pragma unroll
+      for (int k = 0; k < CW; ++k) {
+        const double2 vc = vw[wq][0][c0 + k];
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) acc[q] = cadd(acc[q], zmul(m[q][k], vc));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) part[s & 1][wq][lane + 64 * q] = acc[q];
     __syncthreads();
-    const double2 tot = cadd(cadd(part[0][lane], part[1][lane]), cadd(part[2][lane], part[3][lane]));
-    p = zmul(tau2, tot);
-    if (lane <= s) p = zero;
-    v = vn;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int r = lane + 64 * q;
+      double2 tot = part[s & 1][0][r];
+#pragma unroll
+      for (int u = 1; u < NW; ++u) tot = cadd(tot, part[s & 1][u][r]);
+      p[q] = zmul(tau2, tot);
+      if (r <= s) p[q] = zero;
+      v[q] = vn[q];
+    }
     tau = tau2;
   }
 }
@@ -1495,7 +1580,8 @@ static void enqueue_tridiag(hipStream_t s, int n, double2* A, double2* V, double
     double2* pout = (j & 1) ? p0 : p1;
     const int len = n - j - 1, rows = n - j - 2;
     if (len <= kTailLen) {  // steps j .. n-2 in one launch
-      hipLaunchKernelGGL(k_hetrd_tail, dim3(1), dim3(256), 0, s, A, n, j, V, taus, d, e, pin);
+      hipLaunchKernelGGL((k_hetrd_tail<kTailWaves, kTailLen / 64>), dim3(1), dim3(64 * kTailWaves), 0, s, A, n, j, V,
+                         taus, d, e, pin);
       break;
     }
     const dim3 grid(rows > 0 ? (rows + 3) / 4 : 1);
