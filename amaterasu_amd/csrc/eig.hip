@@ -347,24 +347,44 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A,
 // barrier. Same formulas as k_hetrd_step.
 constexpr int kW64Max = 8;  // Q of the longest wavefront step (trailing length 512)
 
-__device__ __forceinline__ double2 wave_sum2(double2 v) {
-  for (int off = 32; off > 0; off >>= 1) {
-    v.x += __shfl_xor(v.x, off, 64);
-    v.y += __shfl_xor(v.y, off, 64);
-  }
-  return v;
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+__device__ __forceinline__ double lane_f64(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Sum over the wavefront, the same bits in every lane: rotations within each row of 16 lanes
+// (commutative pairs, so every lane of a row ends with the same sum), then the four row sums
+// added in a fixed order.
+__device__ __forceinline__ double wave_allsum(double x) {
+  x += dpp_f64<0x128>(x);  // row_ror:8
+  x += dpp_f64<0x124>(x);  // row_ror:4
+  x += dpp_f64<0x122>(x);  // row_ror:2
+  x += dpp_f64<0x121>(x);  // row_ror:1
+  return (lane_f64(x, 0) + lane_f64(x, 16)) + (lane_f64(x, 32) + lane_f64(x, 48));
+}
+
+__device__ __forceinline__ double2 wave_sum2(double2 v) { return make_double2(wave_allsum(v.x), wave_allsum(v.y)); }
 
 // Element idx (uniform across the wavefront) of a vector distributed as t = lane + 64 q; the
 // element is selected from the registers by unrolled compares (no dynamic register index).
 template <int Q>
 __device__ __forceinline__ double2 w64_element(const double2 (&x)[Q], int idx) {
   const int q = idx >> 6, src = idx & 63;
-  double2 v = x[0];
+  double vx = x[0].x, vy = x[0].y;
 #pragma unroll
-  for (int u = 1; u < Q; ++u)
-    if (q == u) v = x[u];
-  return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+  for (int u = 1; u < Q; ++u) {
+    vx = q == u ? x[u].x : vx;
+    vy = q == u ? x[u].y : vy;
+  }
+  return make_double2(lane_f64(vx, src), lane_f64(vy, src));
 }
 #define AMT_W64(x, idx) w64_element<Q>(x, idx)
 
@@ -491,30 +511,6 @@ __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_wave_barrier();
   __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
   __builtin_amdgcn_wave_barrier();
-}
-
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double lane_f64(double x, int l) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-// Sum over the wavefront, the same bits in every lane: rotations within each row of 16 lanes
-// (commutative pairs, so every lane of a row ends with the same sum), then the four row sums
-// added in a fixed order.
-__device__ __forceinline__ double wave_allsum(double x) {
-  x += dpp_f64<0x128>(x);  // row_ror:8
-  x += dpp_f64<0x124>(x);  // row_ror:4
-  x += dpp_f64<0x122>(x);  // row_ror:2
-  x += dpp_f64<0x121>(x);  // row_ror:1
-  return (lane_f64(x, 0) + lane_f64(x, 16)) + (lane_f64(x, 32) + lane_f64(x, 48));
 }
 
 template <int K>
