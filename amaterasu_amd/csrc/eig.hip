@@ -52,6 +52,31 @@ constexpr int kRRCols = 64;
 constexpr int kTriChunk = 8;
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) { return cmul(a, b); }
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_f64(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Sum over the wavefront, the same bits in every lane: rotations within each row of 16 lanes
+// (commutative pairs, so every lane of a row ends with the same sum), then the four row sums
+// added in a fixed order.
+__device__ __forceinline__ double wave_allsum(double x) {
+  x += dpp_f64<0x128>(x);  // row_ror:8
+  x += dpp_f64<0x124>(x);  // row_ror:4
+  x += dpp_f64<0x122>(x);  // row_ror:2
+  x += dpp_f64<0x121>(x);  // row_ror:1
+  return (lane_f64(x, 0) + lane_f64(x, 16)) + (lane_f64(x, 32) + lane_f64(x, 48));
+}
+
 __device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
 
 // Block-wide sum of a double2 (all threads get the result). red must hold 2 * NT/64 + 2 doubles.
@@ -161,10 +186,7 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_start(const double2* __restri
 // Block-wide sum of a double2 with one barrier: every thread adds the per-wave partials
 // itself, in the same order. red (2 * NT/64 doubles) must not be reused in the same launch.
 __device__ __forceinline__ double2 block_sum2_once(double2 v, double* red) {
-  for (int off = 32; off > 0; off >>= 1) {
-    v.x += __shfl_down(v.x, off, 64);
-    v.y += __shfl_down(v.y, off, 64);
-  }
+  v = make_double2(wave_allsum(v.x), wave_allsum(v.y));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) {
     red[2 * wave] = v.x;
@@ -332,10 +354,7 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A,
     }
     const double2 s1 = zmul(sc, a1);
     double2 acc = make_double2(a0.x + s1.x, a0.y + s1.y);
-    for (int off = 32; off > 0; off >>= 1) {
-      acc.x += __shfl_down(acc.x, off, 64);
-      acc.y += __shfl_down(acc.y, off, 64);
-    }
+    acc = make_double2(wave_allsum(acc.x), wave_allsum(acc.y));
     if (lane == 0) p_out[myrow - j - 2] = zmul(tau2, acc);
   }
 }
@@ -346,30 +365,6 @@ __global__ __launch_bounds__(eig::NT) void k_hetrd_step(double2* __restrict__ A,
 // the lane's own elements only and the three sums are wavefront reductions: no LDS and no
 // barrier. Same formulas as k_hetrd_step.
 constexpr int kW64Max = 8;  // Q of the longest wavefront step (trailing length 512)
-
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double lane_f64(double x, int l) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-// Sum over the wavefront, the same bits in every lane: rotations within each row of 16 lanes
-// (commutative pairs, so every lane of a row ends with the same sum), then the four row sums
-// added in a fixed order.
-__device__ __forceinline__ double wave_allsum(double x) {
-  x += dpp_f64<0x128>(x);  // row_ror:8
-  x += dpp_f64<0x124>(x);  // row_ror:4
-  x += dpp_f64<0x122>(x);  // row_ror:2
-  x += dpp_f64<0x121>(x);  // row_ror:1
-  return (lane_f64(x, 0) + lane_f64(x, 16)) + (lane_f64(x, 32) + lane_f64(x, 48));
-}
 
 __device__ __forceinline__ double2 wave_sum2(double2 v) { return make_double2(wave_allsum(v.x), wave_allsum(v.y)); }
 
@@ -820,10 +815,7 @@ __global__ __launch_bounds__(eig::NT, 2) void k_hetrd_step_wide(double2* __restr
     }
     const double2 s1 = zmul(sc, a1);
     double2 acc = make_double2(a0.x + s1.x, a0.y + s1.y);
-    for (int off = 32; off > 0; off >>= 1) {
-      acc.x += __shfl_down(acc.x, off, 64);
-      acc.y += __shfl_down(acc.y, off, 64);
-    }
+    acc = make_double2(wave_allsum(acc.x), wave_allsum(acc.y));
     if (lane == 0) p_out[myrow - j - 2] = zmul(tau2, acc);
   }
 }
