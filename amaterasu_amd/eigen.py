@@ -91,6 +91,26 @@ def _cholesky_orthonormalize(block: torch.Tensor):
     return _rotate(linv, block)
 
 
+#: Largest |(V V^H - I)_ij| the Loewdin refinement accepts, per block precision: its error is
+#: O(E^3), below the rounding of the block's own arithmetic for E under these bounds.
+_LOWDIN_MAX = {torch.complex64: 2e-3, torch.complex128: 1e-5}
+
+
+def _lowdin_refine(v: torch.Tensor):
+    """Rows of ``v`` (already orthonormal up to E = v v^H - I) made orthonormal to O(E^3) by
+    the series (I + E)^{-1/2} = I - E/2 + 3 E^2 / 8, the second CholeskyQR pass's job without
+    a factorisation; None (one host read, as the Cholesky status would need) when E is too
+    large or not finite, and the caller factorises."""
+    g = _hermitian(_gram(v, v))
+    eye = torch.eye(g.shape[0], dtype=g.dtype, device=g.device)
+    e = g - eye
+    rot = eye - 0.5 * e + 0.375 * (e @ e)
+    err = float(torch.view_as_real(e).abs().max().item())
+    if not err <= _LOWDIN_MAX[v.dtype]:  # also false for NaN
+        return None
+    return _rotate(rot, v)
+
+
 def _cholesky_keeps_all(g: torch.Tensor, threshold: float):
     """L^{-1} when the Cholesky factor proves that every singular value of the block
     exceeds ``threshold`` times the largest (then every direction is kept), else None.
@@ -110,8 +130,9 @@ def _gram_orthonormalize(block: torch.Tensor, threshold: float) -> torch.Tensor:
     """Rows spanning the directions with singular value > threshold * s_max, orthonormal.
 
     When the Cholesky factor of the Gram matrix proves that no singular value falls
-    below the threshold, the block is orthonormalised by two Cholesky passes
-    (CholeskyQR2) and spans the same subspace the SVD route keeps. Otherwise s^2 and the
+    below the threshold, the block is orthonormalised by one Cholesky pass and a Loewdin
+    refinement (or a second Cholesky pass when the first left the rows far from
+    orthonormal: CholeskyQR2) and spans the same subspace the SVD route keeps. Otherwise s^2 and the
     right singular vectors come from the eigen-decomposition of the Gram matrix
     (descending, as an SVD orders them), followed by one CholeskyQR pass.
     """
@@ -119,7 +140,9 @@ def _gram_orthonormalize(block: torch.Tensor, threshold: float) -> torch.Tensor:
     linv = _cholesky_keeps_all(g, threshold)
     if linv is not None:
         v = _rotate(linv, block)
-        again = _cholesky_orthonormalize(v)
+        again = _lowdin_refine(v)
+        if again is None:
+            again = _cholesky_orthonormalize(v)
         return v if again is None else again
     w, u = torch.linalg.eigh(g)
     w = torch.flip(w, dims=[0])
