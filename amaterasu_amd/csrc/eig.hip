@@ -1302,6 +1302,7 @@ struct TriShared {
   double dots[2 * eig::kClusterMax];  // a piece's rows and the previous piece's
   double red[eig::NT / 64 + 1];
   int order[eig::kClusterMax];
+  int flag;
 };
 
 // Workgroup-wide sum of a double (all threads get it).
@@ -1350,6 +1351,128 @@ __device__ bool orthonormalize_row(double* X, int n, int t, TriShared& sh) {
   return true;
 }
 
+// out[a][b] = sum_i A[a n + i] B[b n + i] for a < na, b < nb (both <= kClusterMax): slices of
+// kRRCols columns of both staged in LDS, every thread accumulating its own entries.
+__device__ void slice_dots(const double* A, int na, const double* B, int nb, int n,
+                           double (*out)[eig::kClusterMax + 1]) {
+  constexpr int PPT = (eig::kClusterMax * eig::kClusterMax + eig::NT - 1) / eig::NT;
+  const int tid = threadIdx.x, npair = na * nb;
+  int pa[PPT], pb[PPT];
+  double acc[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int pr = tid + eig::NT * q;
+    pa[q] = pr / nb;
+    pb[q] = pr - pa[q] * nb;
+    acc[q] = 0.0;
+  }
+  double* xs = tri_dyn;  // [kClusterMax][kRRCols + 1]
+  double* ys = tri_dyn + eig::kClusterMax * (kRRCols + 1);
+  for (int i0 = 0; i0 < n; i0 += kRRCols) {
+    __syncthreads();
+    for (int idx = tid; idx < na * kRRCols; idx += eig::NT) {
+      const int t = idx / kRRCols, u = idx - t * kRRCols, i = i0 + u;
+      xs[t * (kRRCols + 1) + u] = i < n ? A[(long long)t * n + i] : 0.0;
+    }
+    for (int idx = tid; idx < nb * kRRCols; idx += eig::NT) {
+      const int t = idx / kRRCols, u = idx - t * kRRCols, i = i0 + u;
+      ys[t * (kRRCols + 1) + u] = i < n ? B[(long long)t * n + i] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      if (tid + eig::NT * q < npair) {
+        const double* xa = xs + pa[q] * (kRRCols + 1);
+        const double* yb = ys + pb[q] * (kRRCols + 1);
+        double sum = 0.0;
+#pragma unroll 8
+        for (int u = 0; u < kRRCols; ++u) sum += xa[u] * yb[u];
+        acc[q] += sum;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PPT; ++q)
+    if (tid + eig::NT * q < npair) out[pa[q]][pb[q]] = acc[q];
+  __syncthreads();
+}
+
+// The rows P of one piece (c rows at X + off n) made orthonormal and orthogonal to the rows
+// X[0, off) of the previous piece, as a block: the projection off the previous piece twice,
+// then CholeskyQR twice (Gram matrix, Cholesky factor L and L^{-1} in LDS on one wavefront,
+// P <- L^{-1} P in place). The same rows Gram-Schmidt would give, in exact arithmetic. Uses
+// no global scratch: the workspace rows beside the piece hold the LU factors that the next
+// inverse-iteration round reads.
+// Returns false when a Cholesky pivot shows a row within 1e-7 (relative) of the span of the
+// rows before it; the caller then takes the row-by-row route, which restarts such rows.
+__device__ bool orthonormalize_block(double* X, int n, int off, int c, TriShared& sh) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  double* P = X + (long long)off * n;
+  for (int rep = 0; rep < 2 && off > 0; ++rep) {
+    slice_dots(P, c, X, off, n, sh.K);  // D = P Q^T
+    for (int idx = tid; idx < c * n; idx += eig::NT) {
+      const int a = idx / n, i = idx - a * n;
+      double v = P[idx];
+      for (int b = 0; b < off; ++b) v -= sh.K[a][b] * X[(long long)b * n + i];
+      P[idx] = v;
+    }
+    __syncthreads();
+  }
+  for (int rep = 0; rep < 2; ++rep) {
+    slice_dots(P, c, P, c, n, sh.K);  // G = P P^T
+    if (wave == 0) {  // lane r owns row r: Cholesky in the lower triangle of K, L^{-1} into Q
+      if (lane < c) sh.dots[lane] = sh.K[lane][lane];
+      bool good = true;
+      wave_lds_order();
+      for (int k = 0; k < c && good; ++k) {
+        const double gkk = sh.K[k][k];
+        if (!(gkk > 1e-14 * sh.dots[k])) {
+          good = false;
+          break;
+        }
+        const double lkk = sqrt(gkk);
+        wave_lds_order();
+        if (lane > k && lane < c) sh.K[lane][k] /= lkk;
+        if (lane == k) sh.K[k][k] = lkk;
+        wave_lds_order();
+        if (lane > k && lane < c) {
+          const double lrk = sh.K[lane][k];
+          for (int m = k + 1; m <= lane; ++m) sh.K[lane][m] -= lrk * sh.K[m][k];
+        }
+        wave_lds_order();
+      }
+      if (good) {
+        for (int r = 0; r < c; ++r) {  // row r of L^{-1}: lane k < r from rows m < r
+          const double lrr = sh.K[r][r];
+          if (lane < r) {
+            double sum = 0.0;
+            for (int m = lane; m < r; ++m) sum += sh.K[r][m] * sh.Q[m][lane];
+            sh.Q[r][lane] = -sum / lrr;
+          } else if (lane == r) {
+            sh.Q[r][r] = 1.0 / lrr;
+          }
+          wave_lds_order();
+        }
+      }
+      if (lane == 0) sh.flag = good ? 1 : 0;
+    }
+    __syncthreads();
+    if (!sh.flag) return false;
+    // P <- L^{-1} P in place: each thread owns whole columns and, L^{-1} being lower
+    // triangular, overwrites row a only after the rows above it no longer need it (a
+    // descending)
+    for (int i = tid; i < n; i += eig::NT) {
+      for (int a = c - 1; a >= 0; --a) {
+        double v = 0.0;
+        for (int b = 0; b <= a; ++b) v += sh.Q[a][b] * P[(long long)b * n + i];
+        P[(long long)a * n + i] = v;
+      }
+    }
+    __syncthreads();
+  }
+  return true;
+}
+
 // One piece of a chain of close eigenvalues, rows [ps, ps + c) of S, in one workgroup. X
 // points at the row `off` before the piece: rows 0..off-1 of X are the previous piece of the
 // same chain (already orthonormal), so the Gram-Schmidt pass makes the piece orthogonal to
@@ -1359,7 +1482,8 @@ __device__ void solve_piece(const double* __restrict__ d, const double* __restri
                             int last, const double* __restrict__ wb, double tnorm, double* __restrict__ w,
                             double* X, int off, int ps, int c, double* __restrict__ tmp, TriShared& sh) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int t = off; t < off + c; ++t) {
+  const bool block_done = c > 1 && orthonormalize_block(X, n, off, c, sh);
+  for (int t = off; t < off + c && !block_done; ++t) {
     for (int attempt = 0; attempt < 4; ++attempt) {
       if (orthonormalize_row(X, n, t, sh)) break;
       // dependent on the earlier rows: restart this row from fresh pseudo-random values
