@@ -618,9 +618,10 @@ def main() -> int:
             "workload": "si64-scf: Si64 2x2x2 conventional (a=5.431 A, seeded displacements sigma 0.05 A), PBE, "
                         "ecut 20 Ha, 84^3, Gamma, 160 bands, SCF to 1e-6 Ha/atom + density residual 1e-6 from the "
                         "learned start; one step = one whole SCF",
-            "precision_policy": "FP32 (FP32 accumulation) FFTs, V_loc, projector and subspace GEMMs; FP64 Cholesky, "
-                                "subspace eigensolve, occupations, energies, mixing; FP64 blocks below density "
-                                "residual 1e-4",
+            "precision_policy": "FP32 Davidson blocks for the whole SCF: FFTs, V_loc, projector and subspace GEMMs "
+                                "in FP32 (FP32 sums over at most 1024 products, those sums added in FP64), "
+                                "eigensolver tolerance floored at 2e-6 while FP32 blocks are used; FP64 Cholesky, "
+                                "subspace eigensolve, occupations, energies, XC, Hartree and mixing",
             "stated_errors_vs_fp64": scf["errors_vs_fp64"],
             "parallelism": "single GPU",
         },
