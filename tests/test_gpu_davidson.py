@@ -110,3 +110,47 @@ def test_davidson_with_gpu_hamiltonian_matches_oracle(si2_system):
         # same occupied subspace: projector difference
         pa, pb = a.coefficients[:4].T @ a.coefficients[:4].conj(), b.coefficients[:4].T @ b.coefficients[:4].conj()
         assert np.abs(pa - pb).max() < 1e-7
+
+
+@pytest.mark.parametrize("dtype,cond_hard,orth", [("complex64", 1e3, 1e-5), ("complex128", 1e6, 1e-12)])
+def test_orthonormalisation_refinement_and_its_fallback(monkeypatch, dtype, cond_hard, orth):
+    """After the first Cholesky pass the rows of a well-conditioned block are orthonormal to
+    within the Loewdin bound and take the refinement (no second factorisation); with the
+    bound at zero the same block takes the second Cholesky pass; a block whose singular
+    values spread over cond_hard goes whichever way its |E| sends it. Every result is
+    orthonormal and spans the input rows."""
+    import torch
+
+    from amaterasu_amd import eigen
+
+    calls = []
+    second = eigen._cholesky_orthonormalize
+    monkeypatch.setattr(eigen, "_cholesky_orthonormalize", lambda v: calls.append(1) or second(v))
+    rng = np.random.default_rng(3)
+    n, m = 4000, 24
+    tdt = getattr(torch, dtype)
+
+    def block_of(cond):
+        u, _ = np.linalg.qr(rng.normal(size=(n, m)) + 1j * rng.normal(size=(n, m)))
+        w, _ = np.linalg.qr(rng.normal(size=(m, m)) + 1j * rng.normal(size=(m, m)))
+        return (w * np.geomspace(1.0, 1.0 / cond, m)) @ u.conj().T  # singular values 1 .. 1/cond
+
+    def check(block, expect_calls):
+        calls.clear()
+        dev = torch.from_numpy(block).to(tdt).cuda()
+        out = eigen._gram_orthonormalize(dev, 1e-7)
+        if expect_calls is not None:
+            assert len(calls) == expect_calls
+        o = out.to(torch.complex128).cpu().numpy()
+        b = dev.to(torch.complex128).cpu().numpy()
+        assert o.shape == (m, n)
+        assert np.abs(o @ o.conj().T - np.eye(m)).max() < orth
+        assert np.abs((b @ o.conj().T) @ o - b).max() < (1e-3 if dtype == "complex64" else 1e-9)
+
+    easy = block_of(2.0)
+    check(easy, 0)
+    monkeypatch.setitem(eigen._LOWDIN_MAX, tdt, 0.0)
+    check(easy, 1)
+    monkeypatch.undo()
+    monkeypatch.setattr(eigen, "_cholesky_orthonormalize", lambda v: calls.append(1) or second(v))
+    check(block_of(cond_hard), None)
